@@ -1,0 +1,244 @@
+"""Benchmark: pair-interactions/s of the DEM step on one MI355X.
+
+Workload: 5000-sphere ballistic particle-cluster aggregate (seeded), lognormal
+radii (median 1e-5 cm, log-sd 0.2), dust constants with cor 0.64 and u_s 0.3,
+gravity on all pairs, velocities isotropic Gaussian (sd 1e-3 cm/s),
+dt = t_c/50; 20k velocity-Verlet steps.  Inputs are resident in HBM before the
+timed region.
+
+A "step" is one sim_one_step over the whole aggregate (drift, all-pairs force
+pass, kick).  pair-interactions/s = steps * N(N-1)/2 / seconds, unique pairs.
+
+Modes: fp64-ordered (bitwise the reference's arithmetic), fp64-fast (Newton-3
+tiles) and mixed (binary32 far field).  ``value`` is the fastest FP64 mode; every
+mode's rate and its energy/position error against fp64-ordered are reported.
+
+    python bench.py [--gpus 1] [--steps 20000] [--warmup 100]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X vector FP64, dense
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+F_FAR = {0: 52, 1: 62}    # algorithmic flops per far-field pair (gravity off / on)
+F_CONTACT = 214           # algorithmic flops per touching pair
+V_CUSTOM = 0.36301555459799423  # dust_const.hpp:34
+SEED = 0x5EED0001
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload(n):
+    import spacelab_amd as sl
+    g = sl.generate_bpca(n, seed=SEED, r_med=1e-5, sigma=0.2, rho=2.25, v_sigma=1e-3)
+    p = sl.calibrate(sl.preset("config"), g.R, g.m, velocity=V_CUSTOM, cor=0.64, dt_rule="tc50")
+    return g, p
+
+
+def run_mode(g, p, order, precision, steps, warmup, prof_steps):
+    import spacelab_amd as sl
+    n = g.num_particles
+    q = p.replace(order=order, precision=precision)
+    with sl.Collider(n, q) as c:
+        c.upload(g)
+        e0 = c.init_conditions()
+        if warmup:
+            c.step(warmup, 0)
+        c.synchronize()
+        t0 = time.perf_counter()
+        c.step(steps, 0)
+        c.synchronize()
+        t1 = time.perf_counter()
+        # second pass with HIP events around each kernel launch on the context's stream
+        c.profile_enable(True, reset=True)
+        ew = c.step(prof_steps, prof_steps)
+        prof = c.profile_read()
+        c.profile_enable(False, reset=False)
+        out = c.download()
+    sec = t1 - t0
+    pairs = n * (n - 1) // 2
+    return {"seconds": sec, "steps": steps, "pairs_per_s": steps * pairs / sec,
+            "ms_per_step": 1e3 * sec / steps, "prof": prof, "prof_steps": prof_steps,
+            "e0": e0, "e_end": ew[-1] if ew else None, "state": out}
+
+
+def cpu_baseline(g, p, target_s=15.0):
+    """The reference's own sim_one_step (oracle/_ref) on a bounded sample of the
+    same workload, one host core; falls back to the oracle port if not built."""
+    from oracle import oracle
+    n = g.num_particles
+    pairs = n * (n - 1) // 2
+    st = oracle.State(g.pos, g.vel, g.w, g.R, g.m, g.moi)
+    # size the sample from a short probe so it takes about target_s
+    if oracle.ref_available():
+        t = time.perf_counter()
+        oracle.run_reference(st, p.dt, p.kin, p.kout, 2, 0)      # init + 1 step
+        probe = (time.perf_counter() - t) / 2
+        k = max(2, min(200, int(target_s / max(probe, 1e-6))))
+        t = time.perf_counter()
+        oracle.run_reference(st, p.dt, p.kin, p.kout, k, 0)      # init + (k-1) steps
+        sec = time.perf_counter() - t
+        return {"value": k * pairs / sec, "unit": "pair-interactions/s", "cores": 1,
+                "kind": "reference",
+                "sample": f"{n}-sphere aggregate, step-0 pass + {k - 1} steps of the reference's "
+                          f"sim_one_step (reference physics: gravity on touching pairs only), "
+                          f"{sec:.1f} s"}
+    P = oracle.Params(G=p.G, Ha=p.Ha, h_min=p.h_min, u_s=p.u_s, u_r=p.u_r, kin=p.kin,
+                      kout=p.kout, dt=p.dt, far_gravity=p.far_gravity)
+    t = time.perf_counter()
+    oracle.run(st, P, 2, 0, mode="tri")
+    probe = (time.perf_counter() - t) / 2
+    k = max(2, min(200, int(target_s / max(probe, 1e-6))))
+    t = time.perf_counter()
+    oracle.run(st, P, k, 0, mode="tri")
+    sec = time.perf_counter() - t
+    return {"value": k * pairs / sec, "unit": "pair-interactions/s", "cores": 1, "kind": "port",
+            "sample": f"{n}-sphere aggregate, step-0 pass + {k - 1} steps of the oracle's "
+                      f"triangular loop, {sec:.1f} s"}
+
+
+def cpu_omp(g, p, threads, target_s=8.0):
+    from oracle import oracle
+    n = g.num_particles
+    pairs = n * (n - 1) // 2
+    st = oracle.State(g.pos, g.vel, g.w, g.R, g.m, g.moi)
+    P = oracle.Params(G=p.G, Ha=p.Ha, h_min=p.h_min, u_s=p.u_s, u_r=p.u_r, kin=p.kin,
+                      kout=p.kout, dt=p.dt, far_gravity=p.far_gravity)
+    t = time.perf_counter()
+    oracle.run(st, P, 2, 0, mode="gather", nthreads=threads)
+    probe = (time.perf_counter() - t) / 2
+    k = max(2, min(400, int(target_s / max(probe, 1e-6))))
+    t = time.perf_counter()
+    oracle.run(st, P, k, 0, mode="gather", nthreads=threads)
+    sec = time.perf_counter() - t
+    # the gather evaluates every pair from both sides; count unique pairs
+    return {"value": k * pairs / sec, "unit": "pair-interactions/s", "cores": threads,
+            "kind": "port", "sample": f"ordered gather, OpenMP {threads} threads, {k} pair passes, {sec:.1f} s"}
+
+
+def energy_scale(e):
+    return abs(e.PE) + e.KE
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=5000)
+    ap.add_argument("--modes", default="ordered,fast,mixed")
+    ap.add_argument("--prof-steps", type=int, default=500)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); this build does not shard "
+              "across GPUs", file=sys.stderr)
+        sys.exit(2)
+
+    import spacelab_amd as sl
+    n = a.n
+    g, p = workload(n)
+    pairs = n * (n - 1) // 2
+    log(f"workload: n={n} pairs/step={pairs} dt={p.dt:.4e} kin={p.kin:.4e} steps={a.steps}")
+
+    modes = {"fp64-ordered": (sl.ORDER_ORDERED, sl.PREC_FP64),
+             "fp64-fast": (sl.ORDER_FAST, sl.PREC_FP64),
+             "mixed": (sl.ORDER_FAST, sl.PREC_MIXED)}
+    wanted = {"ordered": "fp64-ordered", "fast": "fp64-fast", "mixed": "mixed"}
+    res = {}
+    for key in a.modes.split(","):
+        name = wanted[key.strip()]
+        order, prec = modes[name]
+        log(f"mode {name}: running {a.warmup} warmup + {a.steps} timed + {a.prof_steps} profiled steps")
+        try:
+            res[name] = run_mode(g, p, order, prec, a.steps, a.warmup, a.prof_steps)
+        except sl.SpacelabError as e:
+            log(f"mode {name} unavailable: {e}")
+            continue
+        r = res[name]
+        log(f"mode {name}: {r['pairs_per_s']:.4e} pairs/s, {r['ms_per_step']:.4f} ms/step")
+
+    if not res:
+        raise SystemExit("no mode ran")
+    ref = res.get("fp64-ordered")
+    modes_out = {}
+    for name, r in res.items():
+        pr = r["prof"]
+        pair_ms = pr["pair_ms"] / max(pr["pair_launches"], 1)
+        contacts = pr["contacts"] / max(r["prof_steps"], 1)
+        f_far = F_FAR[p.far_gravity]
+        flops = f_far * (pairs - contacts) + F_CONTACT * contacts
+        d = {"pairs_per_s": r["pairs_per_s"], "ms_per_step": r["ms_per_step"],
+             "pair_kernel_ms": pair_ms, "ball_kernel_ms": pr["ball_ms"] / max(pr["ball_launches"], 1),
+             "contacts_per_step": contacts,
+             "pair_kernel_tflops": flops / (pair_ms * 1e-3) / 1e12 if pair_ms > 0 else None}
+        e0, e1 = r["e0"], r["e_end"]
+        if e1 is not None:
+            d["energy_drift_rel"] = abs(e1.E - e0.E) / energy_scale(e0)
+        if ref is not None and name != "fp64-ordered" and ref["e_end"] is not None and e1 is not None:
+            d["energy_err_vs_fp64_ordered"] = abs(e1.E - ref["e_end"].E) / energy_scale(ref["e_end"])
+            dx = np.abs(r["state"].pos - ref["state"].pos).max()
+            d["pos_err_vs_fp64_ordered"] = float(dx / np.abs(ref["state"].pos).max())
+            Lr = ref["e_end"].L
+            d["angmom_err_vs_fp64_ordered"] = float(np.linalg.norm(e1.L - Lr) /
+                                                    max(np.linalg.norm(Lr), 1e-300))
+        modes_out[name] = d
+
+    primary = "fp64-fast" if "fp64-fast" in res else "fp64-ordered"
+    pm = modes_out[primary]
+    r = res[primary]
+    achieved = pm["pair_kernel_tflops"]
+    roof = {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS if achieved else None, "traffic": None,
+            "kernel": "pair pass", "flops_per_far_pair": F_FAR[p.far_gravity],
+            "flops_per_contact_pair": F_CONTACT}
+    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(traffic_file):
+        try:
+            tr = json.load(open(traffic_file)).get(primary)
+            if tr:
+                roof["traffic"] = tr["bytes_per_launch"]
+        except (ValueError, KeyError):
+            pass
+
+    cpu = None
+    extra = {}
+    if not a.no_cpu:
+        log("cpu baseline: reference sim_one_step on one host core (bounded sample)")
+        cpu = cpu_baseline(g, p, a.cpu_seconds)
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: oracle ordered gather on {threads} threads")
+        extra["cpu_omp"] = cpu_omp(g, p, threads)
+
+    line = {"metric": "pair-interactions/s FP64 vs mixed at stated energy error; collisions/s surrogate",
+            "value": r["pairs_per_s"], "unit": "pair-interactions/s", "n_gpus": 1,
+            "steps": a.steps, "warmup": a.warmup, "ms_per_step": r["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": f"plain FP64 vs mixed single aggregate, {n} spheres, seeded BPCA, "
+                                   f"lognormal R (median 1e-5 cm, sigma 0.2), cor 0.64, mu_s 0.3, "
+                                   f"all-pairs gravity, dt=t_c/50, {a.steps} steps",
+                       "n_spheres": n, "pairs_per_step": pairs, "mode": primary,
+                       "parallelism": "single GPU"},
+            "modes": modes_out, "roofline": roof, "cpu_baseline": cpu}
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

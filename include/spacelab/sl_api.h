@@ -1,0 +1,159 @@
+/*
+ * spacelab_amd C ABI -- the MI355X (gfx950) implementation of SpaceLab's per-step
+ * DEM hot path: the Ball_group all-pairs force loop (van der Waals cohesion,
+ * gravity, elastic contact with loading/unloading hysteresis, sliding and rolling
+ * friction) and the velocity-Verlet update with its energy and momentum sums.
+ *
+ * What it replaces in the reference:
+ *   sl_init_forces   <- Ball_group::init_conditions        (ball_group.hpp:777-947)
+ *   sl_step          <- sim_one_step(bool write_step), called once per Step by
+ *                       sim_looper                          (ColliderSingleCore.cpp:78-288, 298-331)
+ *   sl_upload        <- the state a Ball_group holds in pos/vel/w/acc/aacc/R/m/moi
+ *                       (ball_group.hpp:21-53); AoS xyz, so &O.pos[0].x can be passed
+ *   sl_download      <- reading O.pos/O.vel/O.w/O.acc/O.aacc back after a step
+ *   sl_params        <- the constants and globals sim_one_step reads
+ *                       (dust_const.hpp:21-53: G, Ha, h_min, u_s, u_r, kin, kout, dt)
+ *   sl_energy        <- O.PE, O.KE, O.mom, O.ang_mom accumulated on write steps
+ *                       (ColliderSingleCore.cpp:190-204, 228-240, 282-285)
+ *   sl_batch_*       <- no reference counterpart: many independent Ball_groups
+ *                       (collision ensembles) stepped in one launch.
+ *
+ * Conventions: plain C types only; all host pointers are borrowed for the call.
+ * Every call returns 0 on success or a negative SL_E* code, with a message in
+ * sl_last_error(ctx); the library never exits or throws.  A context owns its
+ * device memory and one HIP stream and is used from one thread at a time;
+ * separate contexts may run concurrently.
+ */
+#ifndef SPACELAB_SL_API_H
+#define SPACELAB_SL_API_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SL_API_VERSION 1
+
+/* error codes */
+#define SL_OK 0
+#define SL_E_ARG -1       /* invalid argument (NULL, n out of range, bad parameter) */
+#define SL_E_HIP -2       /* HIP runtime error */
+#define SL_E_STATE -3     /* call out of order (e.g. step before upload) */
+#define SL_E_CAPACITY -4  /* a fixed-capacity table overflowed (near-pair list) */
+#define SL_E_NOGPU -5     /* no usable gfx950 device */
+
+/* precision */
+#define SL_PREC_FP64 0    /* every pair term in binary64 */
+#define SL_PREC_MIXED 1   /* far-field vdW/gravity in binary32 per tile, compensated
+                             binary64 across tiles; contacts and integration binary64 */
+/* order */
+#define SL_ORDER_ORDERED 0 /* bitwise the reference's arithmetic and summation order */
+#define SL_ORDER_FAST 1    /* each pair once (Newton 3), reassociated far-field algebra */
+
+typedef struct sl_ctx sl_ctx;
+
+typedef struct {
+    double G;        /* gravitational constant, cm^3 g^-1 s^-2 */
+    double Ha;       /* Hamaker constant, erg */
+    double h_min;    /* minimum vdW surface separation, cm */
+    double u_s;      /* sliding friction coefficient */
+    double u_r;      /* rolling friction coefficient */
+    double kin;      /* spring constant while loading (distance shrinking) */
+    double kout;     /* spring constant while unloading */
+    double dt;       /* time step, s */
+    int far_gravity; /* 0: gravity on touching pairs only, as the reference codes it
+                        (ColliderSingleCore.cpp:208,227); 1: gravity on all pairs */
+    int precision;   /* SL_PREC_* */
+    int order;       /* SL_ORDER_* */
+} sl_params;
+
+typedef struct {
+    double PE;         /* vdW + elastic (+ gravitational when far_gravity) potential */
+    double KE;         /* translational + rotational kinetic energy */
+    double p[3];       /* total momentum */
+    double L[3];       /* total angular momentum about the origin */
+    int nonfinite;     /* 1 if any position/velocity/spin became Inf/NaN */
+    long long contacts;/* overlapping pairs in this step */
+} sl_energy;
+
+/* Reference presets.  preset 0 = "reference": dust_const.hpp verbatim (cor 0.4,
+ * u_s 0.1, far_gravity 0).  preset 1 = "config": dust_const.hpp with cor 0.64,
+ * u_s 0.3 and far_gravity 1.  kin, kout and dt are left at -1 (set them with
+ * sl_calibrate or by hand). */
+int sl_params_preset(sl_params* out, int preset);
+
+/* kin, kout and dt for a set of radii and masses.
+ *   dt_rule 0: the reference's updateDTK (ball_group.hpp:1313-1329):
+ *              kin = kConsts*r_max*v^2, kout = cor*kin, dt = .01*sqrt(...)
+ *   dt_rule 1: same kin/kout, dt = t_c/50 with t_c = pi*sqrt(m_min/kin), the
+ *              half-period of the contact spring (force (k/2)*overlap). */
+int sl_calibrate(sl_params* p, int n, const double* R, const double* m, double velocity,
+                 double cor, int dt_rule);
+
+int sl_create(sl_ctx** out, int n_max, const sl_params* p);
+int sl_set_params(sl_ctx* ctx, const sl_params* p);
+
+/* Upload a state.  pos3/vel3/w3 are required; acc3/aacc3 may be NULL (zero);
+ * pos_prev3 are the positions the previous pair pass used (NULL: pos3, which
+ * is what they are after any completed step or init pass).  Resets the step
+ * counter to 0. */
+int sl_upload(sl_ctx* ctx, int n, const double* pos3, const double* vel3, const double* w3,
+              const double* acc3, const double* aacc3, const double* R, const double* m,
+              const double* moi, const double* pos_prev3);
+
+/* Step-0 force and energy pass (init_conditions): acc/aacc from the current
+ * positions, energies of the current state. */
+int sl_init_forces(sl_ctx* ctx, sl_energy* e);
+
+/* n_steps calls of sim_one_step.  The context counts steps from the last upload;
+ * a step is a write step when count % write_every == 0 (write_every <= 0: never).
+ * Energies of the write steps are stored in order into e_at_writes (capacity
+ * e_cap, may be NULL).  Returns the number of write steps taken (>= 0) or an
+ * error code. */
+int sl_step(sl_ctx* ctx, int n_steps, int write_every, sl_energy* e_at_writes, int e_cap);
+
+int sl_download(sl_ctx* ctx, double* pos3, double* vel3, double* w3, double* acc3,
+                double* aacc3);
+/* Half-step velocities of the last step (velh, wh in the reference). */
+int sl_download_half(sl_ctx* ctx, double* velh3, double* wh3);
+
+int sl_synchronize(sl_ctx* ctx);
+long long sl_step_count(const sl_ctx* ctx);
+const char* sl_last_error(const sl_ctx* ctx);
+void sl_destroy(sl_ctx* ctx);
+
+/* ---- measurement ---------------------------------------------------------- */
+typedef struct {
+    double pair_ms;        /* summed duration of the pair kernel launches (HIP events) */
+    long long pair_launches;
+    double ball_ms;        /* summed duration of the per-ball kernels */
+    long long ball_launches;
+    long long contacts;    /* overlapping pairs summed over profiled steps */
+    long long near_pairs;  /* pairs routed to the near/contact path (fast mode) */
+} sl_profile;
+
+/* Record HIP events around each kernel launch of sl_step (on the context's
+ * stream) and accumulate; enable=0 stops, reset clears. */
+int sl_profile_enable(sl_ctx* ctx, int enable, int reset);
+int sl_profile_read(sl_ctx* ctx, sl_profile* out);
+
+/* ---- initial conditions --------------------------------------------------- */
+/* Seeded ballistic particle-cluster aggregate: each new sphere is placed touching
+ * a uniformly chosen earlier one in a uniformly random direction, retried until it
+ * overlaps no other sphere.  Radii lognormal (median r_med, log-sd sigma), density
+ * rho, moi = 0.4 m R^2.  Positions are centred on the centre of mass; velocities
+ * are isotropic Gaussian (sd v_sigma per component) with the mass-weighted mean
+ * removed; spins zero.  Deterministic for a seed on every platform. */
+int sl_gen_bpca(int n, unsigned long long seed, double r_med, double sigma, double rho,
+                double v_sigma, double* pos3, double* vel3, double* w3, double* R, double* m,
+                double* moi);
+
+/* ---- arithmetic probe ------------------------------------------------------ */
+/* q = a/b, s = sqrt(a), lg = log(a) element-wise on the device, compiled exactly as
+ * the ordered kernels are.  Lets tests check that binary64 division and square
+ * root are correctly rounded on the device, which bitwise parity relies on. */
+int sl_probe_divsqrt(int n, const double* a, const double* b, double* q, double* s, double* lg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,322 @@
+"""spacelab_amd -- MI355X (gfx950) implementation of SpaceLab's DEM step.
+
+Python binding over the C ABI in ``include/spacelab/sl_api.h`` (library
+``spacelab_amd/lib/libspacelab.so``).  The binding only moves arrays and calls
+the library; every step runs in the HIP kernels.  There is no CPU fallback:
+if the library is missing or no gfx950 GPU is present the calls raise.
+
+The object model mirrors the reference's (ball_group.hpp, ColliderSingleCore.cpp):
+
+* :class:`BallGroup` holds the per-ball arrays of ``Ball_group`` (pos, vel, velh,
+  acc, w, wh, aacc, R, m, moi) as float64 arrays of shape (n, 3) / (n,);
+* :class:`Collider` owns a device context and provides ``init_conditions()``
+  (Ball_group::init_conditions), ``sim_one_step(write_step)`` and
+  ``sim_looper(steps, skip)`` (ColliderSingleCore.cpp:78-331), with the energies
+  of write steps returned as :class:`Energy` records (O.PE, O.KE, O.mom, O.ang_mom).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libspacelab.so")
+
+PREC_FP64, PREC_MIXED = 0, 1
+ORDER_ORDERED, ORDER_FAST = 0, 1
+
+ERRORS = {-1: "SL_E_ARG", -2: "SL_E_HIP", -3: "SL_E_STATE", -4: "SL_E_CAPACITY", -5: "SL_E_NOGPU"}
+
+
+class SpacelabError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    """sl_params: the constants and globals the step reads (dust_const.hpp:21-53)."""
+    _fields_ = [("G", C.c_double), ("Ha", C.c_double), ("h_min", C.c_double),
+                ("u_s", C.c_double), ("u_r", C.c_double), ("kin", C.c_double),
+                ("kout", C.c_double), ("dt", C.c_double), ("far_gravity", C.c_int),
+                ("precision", C.c_int), ("order", C.c_int)]
+
+    def copy(self) -> "Params":
+        p = Params()
+        C.pointer(p)[0] = self
+        return p
+
+    def replace(self, **kw) -> "Params":
+        p = self.copy()
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+class _Energy(C.Structure):
+    _fields_ = [("PE", C.c_double), ("KE", C.c_double), ("p", C.c_double * 3),
+                ("L", C.c_double * 3), ("nonfinite", C.c_int), ("contacts", C.c_longlong)]
+
+
+class _Profile(C.Structure):
+    _fields_ = [("pair_ms", C.c_double), ("pair_launches", C.c_longlong),
+                ("ball_ms", C.c_double), ("ball_launches", C.c_longlong),
+                ("contacts", C.c_longlong), ("near_pairs", C.c_longlong)]
+
+
+@dataclass
+class Energy:
+    """Energies of one write step: O.PE, O.KE, O.mom, O.ang_mom."""
+    PE: float
+    KE: float
+    p: np.ndarray
+    L: np.ndarray
+    nonfinite: bool = False
+    contacts: int = 0
+
+    @property
+    def E(self) -> float:
+        return self.PE + self.KE
+
+    def as_array(self) -> np.ndarray:
+        return np.array([self.PE, self.KE, *self.p, *self.L], dtype=np.float64)
+
+    @staticmethod
+    def _from(e: _Energy) -> "Energy":
+        return Energy(e.PE, e.KE, np.array(e.p[:]), np.array(e.L[:]), bool(e.nonfinite),
+                      int(e.contacts))
+
+
+_lib = None
+
+
+def lib():
+    """Load libspacelab.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m spacelab_amd.build` "
+                          "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    L.sl_params_preset.argtypes = [C.POINTER(Params), C.c_int]
+    L.sl_calibrate.argtypes = [C.POINTER(Params), C.c_int, dp, dp, C.c_double, C.c_double, C.c_int]
+    L.sl_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(Params)]
+    L.sl_set_params.argtypes = [vp, C.POINTER(Params)]
+    L.sl_upload.argtypes = [vp, C.c_int] + [dp] * 9
+    L.sl_init_forces.argtypes = [vp, C.POINTER(_Energy)]
+    L.sl_step.argtypes = [vp, C.c_int, C.c_int, C.POINTER(_Energy), C.c_int]
+    L.sl_download.argtypes = [vp] + [dp] * 5
+    L.sl_download_half.argtypes = [vp, dp, dp]
+    L.sl_synchronize.argtypes = [vp]
+    L.sl_step_count.argtypes = [vp]
+    L.sl_step_count.restype = C.c_longlong
+    L.sl_last_error.argtypes = [vp]
+    L.sl_last_error.restype = C.c_char_p
+    L.sl_destroy.argtypes = [vp]
+    L.sl_destroy.restype = None
+    L.sl_profile_enable.argtypes = [vp, C.c_int, C.c_int]
+    L.sl_profile_read.argtypes = [vp, C.POINTER(_Profile)]
+    L.sl_gen_bpca.argtypes = [C.c_int, C.c_ulonglong, C.c_double, C.c_double, C.c_double,
+                              C.c_double] + [dp] * 6
+    L.sl_probe_divsqrt.argtypes = [C.c_int] + [dp] * 5
+    _lib = L
+    return L
+
+
+def _dp(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a, shape) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def preset(name: str = "reference") -> Params:
+    """'reference': dust_const.hpp verbatim; 'config': cor 0.64, u_s 0.3, all-pairs gravity."""
+    p = Params()
+    rc = lib().sl_params_preset(C.byref(p), {"reference": 0, "config": 1}[name])
+    if rc:
+        raise SpacelabError(rc, "sl_params_preset")
+    return p
+
+
+def calibrate(p: Params, R: np.ndarray, m: np.ndarray, velocity: float, cor: float,
+              dt_rule: str = "tc50") -> Params:
+    """kin, kout and dt (updateDTK, ball_group.hpp:1313-1329, or dt = t_c/50)."""
+    q = p.copy()
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    rc = lib().sl_calibrate(C.byref(q), len(R), _dp(R), _dp(m), velocity, cor,
+                            {"reference": 0, "tc50": 1}[dt_rule])
+    if rc:
+        raise SpacelabError(rc, "sl_calibrate")
+    return q
+
+
+@dataclass
+class BallGroup:
+    """The per-ball arrays of the reference's Ball_group (ball_group.hpp:21-53)."""
+    pos: np.ndarray
+    vel: np.ndarray
+    w: np.ndarray
+    R: np.ndarray
+    m: np.ndarray
+    moi: np.ndarray
+    acc: np.ndarray | None = None
+    aacc: np.ndarray | None = None
+    velh: np.ndarray | None = None
+    wh: np.ndarray | None = None
+
+    @property
+    def num_particles(self) -> int:
+        return int(self.R.shape[0])
+
+    def copy(self) -> "BallGroup":
+        cp = lambda a: None if a is None else np.array(a, dtype=np.float64, copy=True)
+        return BallGroup(cp(self.pos), cp(self.vel), cp(self.w), cp(self.R), cp(self.m),
+                         cp(self.moi), cp(self.acc), cp(self.aacc), cp(self.velh), cp(self.wh))
+
+
+def generate_bpca(n: int, seed: int, r_med: float = 1e-5, sigma: float = 0.2,
+                  rho: float = 2.25, v_sigma: float = 1e-3) -> BallGroup:
+    """Seeded ballistic particle-cluster aggregate (see sl_gen_bpca)."""
+    pos = np.zeros((n, 3)); vel = np.zeros((n, 3)); w = np.zeros((n, 3))
+    R = np.zeros(n); m = np.zeros(n); moi = np.zeros(n)
+    rc = lib().sl_gen_bpca(n, seed, r_med, sigma, rho, v_sigma, _dp(pos), _dp(vel), _dp(w),
+                           _dp(R), _dp(m), _dp(moi))
+    if rc:
+        raise SpacelabError(rc, "sl_gen_bpca")
+    return BallGroup(pos, vel, w, R, m, moi)
+
+
+class Collider:
+    """One simulated Ball_group on the GPU (sim_one_step / sim_looper semantics)."""
+
+    def __init__(self, n_max: int, params: Params):
+        self._ctx = C.c_void_p()
+        rc = lib().sl_create(C.byref(self._ctx), int(n_max), C.byref(params))
+        if rc:
+            raise SpacelabError(rc, "sl_create: no usable gfx950 device" if rc == -5
+                                else "sl_create")
+        self.params = params.copy()
+        self.n = 0
+
+    def _check(self, rc: int, what: str) -> int:
+        if rc < 0:
+            msg = lib().sl_last_error(self._ctx)
+            raise SpacelabError(rc, f"{what}: {msg.decode() if msg else ''}")
+        return rc
+
+    def close(self):
+        if self._ctx:
+            lib().sl_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_params(self, params: Params):
+        self._check(lib().sl_set_params(self._ctx, C.byref(params)), "sl_set_params")
+        self.params = params.copy()
+
+    def upload(self, g: BallGroup, pos_prev: np.ndarray | None = None):
+        n = g.num_particles
+        s3 = (n, 3)
+        pos, vel, w = _f64(g.pos, s3), _f64(g.vel, s3), _f64(g.w, s3)
+        acc = None if g.acc is None else _f64(g.acc, s3)
+        aacc = None if g.aacc is None else _f64(g.aacc, s3)
+        R, m, moi = _f64(g.R, (n,)), _f64(g.m, (n,)), _f64(g.moi, (n,))
+        pp = None if pos_prev is None else _f64(pos_prev, s3)
+        self._check(lib().sl_upload(self._ctx, n, _dp(pos), _dp(vel), _dp(w), _dp(acc), _dp(aacc),
+                                    _dp(R), _dp(m), _dp(moi), _dp(pp)), "sl_upload")
+        self.n = n
+        self._static = (R.copy(), m.copy(), moi.copy())
+
+    def init_conditions(self) -> Energy:
+        e = _Energy()
+        self._check(lib().sl_init_forces(self._ctx, C.byref(e)), "sl_init_forces")
+        return Energy._from(e)
+
+    def step(self, n_steps: int, write_every: int = 0) -> list[Energy]:
+        """n_steps sim_one_step calls; write step when step_count % write_every == 0."""
+        if write_every > 0:
+            c0 = self.step_count
+            cap = (c0 + n_steps) // write_every - c0 // write_every
+        else:
+            cap = 0
+        buf = (_Energy * max(cap, 1))()
+        nw = self._check(lib().sl_step(self._ctx, int(n_steps), int(write_every), buf, max(cap, 1)),
+                         "sl_step")
+        return [Energy._from(buf[k]) for k in range(min(nw, cap))]
+
+    def sim_one_step(self, write_step: bool) -> Energy | None:
+        """One step (ColliderSingleCore.cpp:78-288); energies when write_step."""
+        if write_step:
+            e = _Energy()
+            # write_every = 1 makes this one step a write step
+            self._check(lib().sl_step(self._ctx, 1, 1, C.byref(e), 1), "sl_step")
+            return Energy._from(e)
+        self._check(lib().sl_step(self._ctx, 1, 0, None, 0), "sl_step")
+        return None
+
+    def sim_looper(self, steps: int, skip: int) -> list[Energy]:
+        """Steps 1..steps-1 with a write every skip steps (ColliderSingleCore.cpp:298-331)."""
+        return self.step(steps - 1, skip)
+
+    @property
+    def step_count(self) -> int:
+        return int(lib().sl_step_count(self._ctx))
+
+    def download(self) -> BallGroup:
+        n = self.n
+        pos = np.empty((n, 3)); vel = np.empty((n, 3)); w = np.empty((n, 3))
+        acc = np.empty((n, 3)); aacc = np.empty((n, 3))
+        velh = np.empty((n, 3)); wh = np.empty((n, 3))
+        self._check(lib().sl_download(self._ctx, _dp(pos), _dp(vel), _dp(w), _dp(acc), _dp(aacc)),
+                    "sl_download")
+        self._check(lib().sl_download_half(self._ctx, _dp(velh), _dp(wh)), "sl_download_half")
+        R, m, moi = self._static
+        return BallGroup(pos, vel, w, R.copy(), m.copy(), moi.copy(), acc, aacc, velh, wh)
+
+    def synchronize(self):
+        self._check(lib().sl_synchronize(self._ctx), "sl_synchronize")
+
+    def profile_enable(self, enable: bool = True, reset: bool = True):
+        self._check(lib().sl_profile_enable(self._ctx, int(enable), int(reset)), "sl_profile_enable")
+
+    def profile_read(self) -> dict:
+        p = _Profile()
+        self._check(lib().sl_profile_read(self._ctx, C.byref(p)), "sl_profile_read")
+        return {k: getattr(p, k) for k, _ in p._fields_}
+
+
+def probe_divsqrt(a: np.ndarray, b: np.ndarray):
+    """Device a/b, sqrt(a), log(a) with the ordered kernels' compilation."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    q = np.empty_like(a); s = np.empty_like(a); lg = np.empty_like(a)
+    rc = lib().sl_probe_divsqrt(len(a), _dp(a), _dp(b), _dp(q), _dp(s), _dp(lg))
+    if rc:
+        raise SpacelabError(rc, "sl_probe_divsqrt")
+    return q, s, lg

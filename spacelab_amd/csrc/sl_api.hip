@@ -1,0 +1,535 @@
+// C ABI of the spacelab_amd library: contexts, state transfer, stepping.
+//
+// The step sequence mirrors sim_looper/sim_one_step (ColliderSingleCore.cpp:
+// 298-331, 78-288): drift -> pair pass -> closing kick (+ energies on write
+// steps).  All kernels of a context run in order on its one stream.  Energies
+// are read back only on write steps.
+#include "spacelab/sl_api.h"
+
+#include "sl_common.hpp"
+#include "sl_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace sl;
+
+struct sl_ctx {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    int n_max = 0, n = 0, n_pad = 0;
+    sl_params p{};
+    double* base = nullptr;     // NFIELD * n_pad doubles
+    State st{};
+    double* pe_part = nullptr;  // per-block PE partials
+    double* e_part = nullptr;   // per-block KE/p/L partials
+    unsigned long long* cnt = nullptr;  // [0] contacts total, [1] contacts this write step
+    int* nonfinite = nullptr;
+    double* h_pe = nullptr;     // pinned
+    double* h_e = nullptr;      // pinned
+    unsigned long long* h_cnt = nullptr;
+    int* h_nonfinite = nullptr;
+    FastBuffers fb{};
+    long long step = 0;
+    bool uploaded = false;
+    bool forces_valid = false;
+    std::string err;
+    // profiling
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    struct Mark { int kind; size_t e0, e1; };
+    std::vector<Mark> marks;
+    sl_profile acc_prof{};
+};
+
+namespace {
+
+int set_err(sl_ctx* c, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                     \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return set_err((c), SL_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+int check_params(sl_ctx* c, const sl_params* p)
+{
+    if (!p) return set_err(c, SL_E_ARG, "params is NULL");
+    if (!(p->dt > 0) || !std::isfinite(p->dt)) return set_err(c, SL_E_ARG, "dt must be > 0 (got %g)", p->dt);
+    if (!(p->kin >= 0) || !(p->kout >= 0)) return set_err(c, SL_E_ARG, "kin/kout must be >= 0");
+    if (!(p->h_min > 0)) return set_err(c, SL_E_ARG, "h_min must be > 0");
+    if (p->precision != SL_PREC_FP64 && p->precision != SL_PREC_MIXED)
+        return set_err(c, SL_E_ARG, "unknown precision %d", p->precision);
+    if (p->order != SL_ORDER_ORDERED && p->order != SL_ORDER_FAST)
+        return set_err(c, SL_E_ARG, "unknown order %d", p->order);
+    if (p->precision == SL_PREC_MIXED && p->order == SL_ORDER_ORDERED)
+        return set_err(c, SL_E_ARG, "mixed precision requires SL_ORDER_FAST");
+    return SL_OK;
+}
+
+Phys phys_of(const sl_params& p)
+{
+    Phys P;
+    P.G = p.G;
+    P.Ha = p.Ha;
+    P.h_min = p.h_min;
+    P.u_s = p.u_s;
+    P.u_r = p.u_r;
+    P.kin = p.kin;
+    P.kout = p.kout;
+    P.dt = p.dt;
+    P.far_gravity = p.far_gravity;
+    return P;
+}
+
+hipEvent_t next_event(sl_ctx* c)
+{
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+// Record an event pair around a launch when profiling.
+struct ProfScope {
+    sl_ctx* c;
+    int kind;
+    size_t e0 = 0;
+    bool on = false;
+    ProfScope(sl_ctx* c_, int kind_) : c(c_), kind(kind_)
+    {
+        if (c->prof) {
+            hipEvent_t e = next_event(c);
+            if (e && hipEventRecord(e, c->stream) == hipSuccess) {
+                e0 = c->ev_used - 1;
+                on = true;
+            }
+        }
+    }
+    ~ProfScope()
+    {
+        if (on) {
+            hipEvent_t e = next_event(c);
+            if (e && hipEventRecord(e, c->stream) == hipSuccess) c->marks.push_back({kind, e0, c->ev_used - 1});
+        }
+    }
+};
+
+int flush_profile(sl_ctx* c)
+{
+    if (c->marks.empty()) {
+        c->ev_used = 0;
+        return SL_OK;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (const auto& mk : c->marks) {
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev_pool[mk.e0], c->ev_pool[mk.e1]));
+        if (mk.kind == 0) {
+            c->acc_prof.pair_ms += ms;
+            c->acc_prof.pair_launches++;
+        } else {
+            c->acc_prof.ball_ms += ms;
+            c->acc_prof.ball_launches++;
+        }
+    }
+    c->marks.clear();
+    c->ev_used = 0;
+    return SL_OK;
+}
+
+int free_ctx_buffers(sl_ctx* c)
+{
+    if (c->base) hipFree(c->base);
+    if (c->pe_part) hipFree(c->pe_part);
+    if (c->e_part) hipFree(c->e_part);
+    if (c->cnt) hipFree(c->cnt);
+    if (c->nonfinite) hipFree(c->nonfinite);
+    if (c->h_pe) hipHostFree(c->h_pe);
+    if (c->h_e) hipHostFree(c->h_e);
+    if (c->h_cnt) hipHostFree(c->h_cnt);
+    if (c->h_nonfinite) hipHostFree(c->h_nonfinite);
+    fast_free(c->fb);
+    c->base = nullptr;
+    return SL_OK;
+}
+
+// Read back and sum the partials of a write step (fixed order: deterministic).
+int read_energy(sl_ctx* c, int n_pe_blocks, int n_e_blocks, sl_energy* out, bool has_pe)
+{
+    if (has_pe && n_pe_blocks > 0)
+        HIPCHK(c, hipMemcpyAsync(c->h_pe, c->pe_part, sizeof(double) * n_pe_blocks, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_e, c->e_part, sizeof(double) * 7 * n_e_blocks, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->cnt, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_nonfinite, c->nonfinite, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    sl_energy e{};
+    if (has_pe)
+        for (int b = 0; b < n_pe_blocks; b++) e.PE += c->h_pe[b];
+    for (int b = 0; b < n_e_blocks; b++) {
+        e.KE += c->h_e[7 * b + 0];
+        e.p[0] += c->h_e[7 * b + 1];
+        e.p[1] += c->h_e[7 * b + 2];
+        e.p[2] += c->h_e[7 * b + 3];
+        e.L[0] += c->h_e[7 * b + 4];
+        e.L[1] += c->h_e[7 * b + 5];
+        e.L[2] += c->h_e[7 * b + 6];
+    }
+    e.contacts = (long long)c->h_cnt[1];
+    e.nonfinite = *c->h_nonfinite;
+    if (out) *out = e;
+    return SL_OK;
+}
+
+// One pair pass (+ per-ball pass) for the current positions.
+int pair_pass(sl_ctx* c, bool write, bool kick, int* n_pe_blocks)
+{
+    const Phys P = phys_of(c->p);
+    if (write) HIPCHK(c, launch_zero_u64(c->cnt + 1, 1, c->stream));
+    if (c->p.order == SL_ORDER_ORDERED) {
+        {
+            ProfScope ps(c, 0);
+            HIPCHK(c, launch_pairs_ordered(c->st, P, c->n, write ? 1 : 0, c->pe_part, c->cnt, c->stream));
+        }
+        *n_pe_blocks = ordered_blocks(c->n, nullptr);
+        {
+            ProfScope ps(c, 1);
+            HIPCHK(c, launch_kick(c->st, c->n, c->p.dt, write ? 1 : 0, kick ? 1 : 0, c->e_part, c->nonfinite, c->stream));
+        }
+    } else {
+        int rc;
+        {
+            ProfScope ps(c, 0);
+            rc = fast_pairs(c->fb, c->st, P, c->n, c->p.precision, write, c->cnt, c->stream, &c->err);
+            if (rc) return rc;
+        }
+        {
+            ProfScope ps(c, 1);
+            rc = fast_ball(c->fb, c->st, P, c->n, c->p.precision, write, kick, c->pe_part, c->e_part,
+                           c->cnt, c->nonfinite, c->stream, n_pe_blocks, &c->err);
+            if (rc) return rc;
+        }
+    }
+    return SL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sl_params_preset(sl_params* out, int preset)
+{
+    if (!out) return SL_E_ARG;
+    sl_params p{};
+    // dust_const.hpp:21-39
+    p.G = 6.67e-8;
+    p.Ha = 4.7e-12;
+    p.h_min = 1e-5 * .1;
+    p.u_s = 0.1;
+    p.u_r = 1e-5;
+    p.kin = -1;
+    p.kout = -1;
+    p.dt = -1;
+    p.far_gravity = 0;
+    p.precision = SL_PREC_FP64;
+    p.order = SL_ORDER_ORDERED;
+    if (preset == 1) {
+        p.u_s = 0.3;
+        p.far_gravity = 1;
+    } else if (preset != 0) {
+        return SL_E_ARG;
+    }
+    *out = p;
+    return SL_OK;
+}
+
+int sl_create(sl_ctx** out, int n_max, const sl_params* p)
+{
+    if (!out) return SL_E_ARG;
+    *out = nullptr;
+    if (n_max < 1 || n_max > (1 << 24)) return SL_E_ARG;
+    sl_ctx* c = new (std::nothrow) sl_ctx();
+    if (!c) return SL_E_ARG;
+    int rc = check_params(c, p);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        delete c;
+        return SL_E_NOGPU;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&c->dev) != hipSuccess || hipGetDeviceProperties(&prop, c->dev) != hipSuccess ||
+        std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        delete c;
+        return SL_E_NOGPU;
+    }
+    c->p = *p;
+    c->n_max = n_max;
+    c->n_pad = (n_max + 255) / 256 * 256;
+    auto fail = [&](int code) {
+        free_ctx_buffers(c);
+        if (c->stream) hipStreamDestroy(c->stream);
+        delete c;
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(SL_E_HIP);
+    if (hipMalloc(&c->base, sizeof(double) * (size_t)NFIELD * c->n_pad) != hipSuccess) return fail(SL_E_HIP);
+    const int max_blocks = std::max(ordered_blocks(n_max, nullptr), fast_max_blocks(n_max)) + 8;
+    if (hipMalloc(&c->pe_part, sizeof(double) * max_blocks) != hipSuccess) return fail(SL_E_HIP);
+    if (hipMalloc(&c->e_part, sizeof(double) * 7 * (kick_blocks(n_max) + 8)) != hipSuccess) return fail(SL_E_HIP);
+    if (hipMalloc(&c->cnt, sizeof(unsigned long long) * 2) != hipSuccess) return fail(SL_E_HIP);
+    if (hipMalloc(&c->nonfinite, sizeof(int)) != hipSuccess) return fail(SL_E_HIP);
+    if (hipHostMalloc(&c->h_pe, sizeof(double) * max_blocks, 0) != hipSuccess) return fail(SL_E_HIP);
+    if (hipHostMalloc(&c->h_e, sizeof(double) * 7 * (kick_blocks(n_max) + 8), 0) != hipSuccess) return fail(SL_E_HIP);
+    if (hipHostMalloc(&c->h_cnt, sizeof(unsigned long long) * 2, 0) != hipSuccess) return fail(SL_E_HIP);
+    if (hipHostMalloc(&c->h_nonfinite, sizeof(int), 0) != hipSuccess) return fail(SL_E_HIP);
+    if (fast_alloc(c->fb, n_max, c->n_pad) != 0) return fail(SL_E_HIP);
+    for (int f = 0; f < NFIELD; f++) c->st.f[f] = c->base + (size_t)f * c->n_pad;
+    if (hipMemsetAsync(c->base, 0, sizeof(double) * (size_t)NFIELD * c->n_pad, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->cnt, 0, sizeof(unsigned long long) * 2, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->nonfinite, 0, sizeof(int), c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return fail(SL_E_HIP);
+    *out = c;
+    return SL_OK;
+}
+
+int sl_set_params(sl_ctx* c, const sl_params* p)
+{
+    if (!c) return SL_E_ARG;
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    const bool order_changed = p->order != c->p.order || p->precision != c->p.precision;
+    c->p = *p;
+    if (order_changed && c->uploaded) {
+        // the fast path keeps its own state buffers; bring them in line
+        rc = fast_sync_from(c->fb, c->st, c->n, c->stream, &c->err);
+        if (rc) return rc;
+    }
+    return SL_OK;
+}
+
+int sl_upload(sl_ctx* c, int n, const double* pos3, const double* vel3, const double* w3,
+              const double* acc3, const double* aacc3, const double* R, const double* m,
+              const double* moi, const double* pos_prev3)
+{
+    if (!c) return SL_E_ARG;
+    if (n < 1 || n > c->n_max) return set_err(c, SL_E_ARG, "n=%d outside [1, %d]", n, c->n_max);
+    if (!pos3 || !vel3 || !w3 || !R || !m || !moi) return set_err(c, SL_E_ARG, "required array is NULL");
+    for (int i = 0; i < n; i++) {
+        if (!(R[i] > 0) || !std::isfinite(R[i])) return set_err(c, SL_E_ARG, "R[%d] = %g must be > 0", i, R[i]);
+        if (!(m[i] > 0) || !std::isfinite(m[i])) return set_err(c, SL_E_ARG, "m[%d] = %g must be > 0", i, m[i]);
+        if (!(moi[i] > 0) || !std::isfinite(moi[i])) return set_err(c, SL_E_ARG, "moi[%d] = %g must be > 0", i, moi[i]);
+        for (int k = 0; k < 3; k++)
+            if (!std::isfinite(pos3[3 * i + k]) || !std::isfinite(vel3[3 * i + k]) || !std::isfinite(w3[3 * i + k]))
+                return set_err(c, SL_E_ARG, "non-finite position/velocity/spin at ball %d", i);
+    }
+    const size_t np = (size_t)c->n_pad;
+    std::vector<double> h((size_t)NFIELD * np, 0.0);
+    auto put3 = [&](int f0, const double* a) {
+        if (!a) return;
+        for (int i = 0; i < n; i++) {
+            h[(size_t)(f0 + 0) * np + i] = a[3 * i + 0];
+            h[(size_t)(f0 + 1) * np + i] = a[3 * i + 1];
+            h[(size_t)(f0 + 2) * np + i] = a[3 * i + 2];
+        }
+    };
+    put3(PX, pos3);
+    put3(VX, vel3);
+    put3(WX, w3);
+    put3(AX, acc3);
+    put3(BX, aacc3);
+    put3(HX, vel3);
+    put3(GX, w3);
+    put3(QX, pos_prev3 ? pos_prev3 : pos3);
+    for (int i = 0; i < n; i++) {
+        h[(size_t)RAD * np + i] = R[i];
+        h[(size_t)MASS * np + i] = m[i];
+        h[(size_t)MOI * np + i] = moi[i];
+    }
+    // padding balls: unit radius and mass far away are never read (loops stop at n)
+    HIPCHK(c, hipMemcpyAsync(c->base, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->n = n;
+    c->step = 0;
+    c->uploaded = true;
+    c->forces_valid = (acc3 != nullptr);
+    int rc = fast_sync_from(c->fb, c->st, n, c->stream, &c->err);
+    if (rc) return rc;
+    return SL_OK;
+}
+
+int sl_init_forces(sl_ctx* c, sl_energy* e)
+{
+    if (!c) return SL_E_ARG;
+    if (!c->uploaded) return set_err(c, SL_E_STATE, "init before upload");
+    if (c->p.order == SL_ORDER_ORDERED) {
+        HIPCHK(c, launch_copy_prev(c->st, c->n, c->stream));
+    } else {
+        int rc = fast_begin_init(c->fb, c->st, c->n, c->stream, &c->err);
+        if (rc) return rc;
+    }
+    HIPCHK(c, launch_zero_i32(c->nonfinite, 1, c->stream));
+    int npe = 0;
+    int rc = pair_pass(c, true, false, &npe);
+    if (rc) return rc;
+    rc = read_energy(c, npe, kick_blocks(c->n), e, true);
+    if (rc) return rc;
+    c->forces_valid = true;
+    return flush_profile(c);
+}
+
+int sl_step(sl_ctx* c, int n_steps, int write_every, sl_energy* e_out, int e_cap)
+{
+    if (!c) return SL_E_ARG;
+    if (!c->uploaded) return set_err(c, SL_E_STATE, "step before upload");
+    if (n_steps < 0) return set_err(c, SL_E_ARG, "n_steps < 0");
+    if (!c->forces_valid) return set_err(c, SL_E_STATE, "no forces: call sl_init_forces or upload acc");
+    int nw = 0;
+    const bool fast = c->p.order == SL_ORDER_FAST;
+    for (int k = 0; k < n_steps; k++) {
+        c->step++;
+        const bool write = write_every > 0 && (c->step % write_every) == 0;
+        if (fast) {
+            int rc = fast_drift(c->fb, c->st, c->n, c->p.dt, c->stream, &c->err);
+            if (rc) return rc;
+        } else {
+            HIPCHK(c, launch_drift(c->st, c->n, c->p.dt, c->stream));
+        }
+        int npe = 0;
+        int rc = pair_pass(c, write, true, &npe);
+        if (rc) return rc;
+        if (write) {
+            sl_energy e;
+            rc = read_energy(c, npe, kick_blocks(c->n), &e, true);
+            if (rc) return rc;
+            if (e_out && nw < e_cap) e_out[nw] = e;
+            nw++;
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc = flush_profile(c);
+    if (rc) return rc;
+    return nw;
+}
+
+int sl_download(sl_ctx* c, double* pos3, double* vel3, double* w3, double* acc3, double* aacc3)
+{
+    if (!c) return SL_E_ARG;
+    if (!c->uploaded) return set_err(c, SL_E_STATE, "download before upload");
+    if (c->p.order == SL_ORDER_FAST) {
+        int rc = fast_sync_to(c->fb, c->st, c->n, c->stream, &c->err);
+        if (rc) return rc;
+    }
+    const size_t np = (size_t)c->n_pad;
+    std::vector<double> h((size_t)NFIELD * np);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->base, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    auto get3 = [&](int f0, double* a) {
+        if (!a) return;
+        for (int i = 0; i < c->n; i++) {
+            a[3 * i + 0] = h[(size_t)(f0 + 0) * np + i];
+            a[3 * i + 1] = h[(size_t)(f0 + 1) * np + i];
+            a[3 * i + 2] = h[(size_t)(f0 + 2) * np + i];
+        }
+    };
+    get3(PX, pos3);
+    get3(VX, vel3);
+    get3(WX, w3);
+    get3(AX, acc3);
+    get3(BX, aacc3);
+    return SL_OK;
+}
+
+int sl_download_half(sl_ctx* c, double* velh3, double* wh3)
+{
+    if (!c) return SL_E_ARG;
+    if (!c->uploaded) return set_err(c, SL_E_STATE, "download before upload");
+    if (c->p.order == SL_ORDER_FAST) {
+        int rc = fast_sync_to(c->fb, c->st, c->n, c->stream, &c->err);
+        if (rc) return rc;
+    }
+    const size_t np = (size_t)c->n_pad;
+    std::vector<double> h(6 * np);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->st.f[HX], sizeof(double) * 6 * np, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->n; i++)
+        for (int k = 0; k < 3; k++) {
+            if (velh3) velh3[3 * i + k] = h[(size_t)k * np + i];
+            if (wh3) wh3[3 * i + k] = h[(size_t)(3 + k) * np + i];
+        }
+    return SL_OK;
+}
+
+int sl_synchronize(sl_ctx* c)
+{
+    if (!c) return SL_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SL_OK;
+}
+
+long long sl_step_count(const sl_ctx* c) { return c ? c->step : -1; }
+
+const char* sl_last_error(const sl_ctx* c) { return c ? c->err.c_str() : "NULL context"; }
+
+void sl_destroy(sl_ctx* c)
+{
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto e : c->ev_pool) hipEventDestroy(e);
+    free_ctx_buffers(c);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int sl_profile_enable(sl_ctx* c, int enable, int reset)
+{
+    if (!c) return SL_E_ARG;
+    int rc = flush_profile(c);
+    if (rc) return rc;
+    c->prof = enable != 0;
+    if (reset) {
+        c->acc_prof = sl_profile{};
+        HIPCHK(c, launch_zero_u64(c->cnt, 1, c->stream));
+        HIPCHK(c, fast_reset_counters(c->fb, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return SL_OK;
+}
+
+int sl_profile_read(sl_ctx* c, sl_profile* out)
+{
+    if (!c || !out) return SL_E_ARG;
+    int rc = flush_profile(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->cnt, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = c->acc_prof;
+    out->contacts = (long long)c->h_cnt[0];
+    out->near_pairs = fast_read_near_total(c->fb, c->stream);
+    return SL_OK;
+}
+
+}  // extern "C"

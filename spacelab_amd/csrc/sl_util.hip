@@ -1,0 +1,71 @@
+// Small device utilities.  Counters are cleared by kernels rather than
+// hipMemsetAsync so the same sequence is safe to capture in a HIP graph.
+#include "spacelab/sl_api.h"
+#include "sl_internal.hpp"
+
+namespace sl {
+
+__global__ void k_zero_u64(unsigned long long* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void k_zero_i32(int* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    k_zero_u64<<<(n + 255) / 256, 256, 0, st>>>(p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_i32(int* p, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    k_zero_i32<<<(n + 255) / 256, 256, 0, st>>>(p, n);
+    return hipGetLastError();
+}
+
+}  // namespace sl
+
+namespace sl {
+
+// Arithmetic probe: the binary64 division, square root and logarithm exactly as
+// the ordered kernels issue them (compiled with the same flags).
+__global__ void k_probe(int n, const double* a, const double* b, double* q, double* s, double* lg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    q[i] = a[i] / b[i];
+    s[i] = __builtin_sqrt(a[i]);
+    lg[i] = ::log(a[i]);
+}
+
+}  // namespace sl
+
+extern "C" int sl_probe_divsqrt(int n, const double* a, const double* b, double* q, double* s, double* lg)
+{
+    if (n < 1 || !a || !b || !q || !s || !lg) return SL_E_ARG;
+    double* d = nullptr;
+    const size_t bytes = sizeof(double) * (size_t)n;
+    if (hipMalloc(&d, 5 * bytes) != hipSuccess) return SL_E_HIP;
+    int rc = SL_OK;
+    if (hipMemcpy(d, a, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d + n, b, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = SL_E_HIP;
+    } else {
+        sl::k_probe<<<(n + 255) / 256, 256>>>(n, d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n, d + 4 * (size_t)n);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(q, d + 2 * (size_t)n, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(s, d + 3 * (size_t)n, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(lg, d + 4 * (size_t)n, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = SL_E_HIP;
+    }
+    hipFree(d);
+    return rc;
+}

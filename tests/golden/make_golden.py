@@ -1,0 +1,153 @@
+"""Write the golden vectors for the DEM step from the reference's own code.
+
+Runs ``oracle/_ref/ref_harness`` -- the reference's ``sim_one_step``
+(ColliderSingleCore/ColliderSingleCore.cpp:78-288) compiled from its sources by
+``oracle/Makefile`` -- on small seeded inputs and stores inputs and outputs as
+``tests/golden/<case>.npz`` (plain float64 arrays, no pickles).  Only runs where
+the reference sources are present; the committed fixtures are what the tests
+read.
+
+    python tests/golden/make_golden.py
+
+Cases follow the golden recipe for this path: two spheres head-on (loading then
+unloading), two spheres oblique with spin (sliding and rolling friction), three
+spheres of unequal radii, and 16- and 64-sphere touching clusters; 1, 10 and 100
+steps; write and non-write steps.  Physics constants are the reference's
+compile-time ones (dust_const.hpp:21-41); dt, kin and kout are set per case.
+"""
+from __future__ import annotations
+
+import math
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+DENSITY = 2.25  # dust_const.hpp:22
+
+
+def mass(R):
+    return DENSITY * 4.0 / 3.0 * math.pi * R ** 3
+
+
+def moi(R, m):
+    return 0.4 * m * R * R  # ball_group.hpp:573
+
+
+def spring(R_max, R_min, v=0.36301555459799423, cor=0.4):
+    """kin, kout as updateDTK sets them (ball_group.hpp:1316-1317) and dt = t_c/50."""
+    four_thirds_pi_rho = 4.0 / 3.0 * math.pi * DENSITY
+    k_consts = 3e3 * four_thirds_pi_rho / (0.1 * 0.1)
+    kin = k_consts * R_max * v * v
+    kout = cor * kin
+    t_c = math.pi * math.sqrt(mass(R_min) / kin)
+    return kin, kout, t_c / 50.0
+
+
+def touching_cluster(n, rng, r_med=1e-5, sigma=0.2):
+    """Small ballistic aggregate: each sphere touches a random earlier one."""
+    R = r_med * np.exp(sigma * rng.standard_normal(n))
+    pos = np.zeros((n, 3))
+    for i in range(1, n):
+        for _ in range(1000):
+            j = rng.integers(0, i)
+            d = rng.standard_normal(3)
+            d /= np.linalg.norm(d)
+            p = pos[j] + d * (R[i] + R[j])
+            dd = np.linalg.norm(pos[:i] - p, axis=1) - (R[:i] + R[i])
+            dd[j] = 1.0
+            if np.all(dd >= 0):
+                pos[i] = p
+                break
+        else:
+            raise RuntimeError("cluster placement failed")
+    m = np.array([mass(r) for r in R])
+    pos -= (m[:, None] * pos).sum(0) / m.sum()
+    return pos, R, m
+
+
+def case_two_headon():
+    R = np.array([1e-5, 1e-5])
+    m = np.array([mass(r) for r in R])
+    kin, kout, dt = spring(R.max(), R.min())
+    v = 20.0
+    gap = 2 * v * dt * 4
+    pos = np.array([[-(R[0] + gap / 2), 0, 0], [R[1] + gap / 2, 0, 0]])
+    vel = np.array([[v, 0, 0], [-v, 0, 0]])
+    w = np.zeros((2, 3))
+    return pos, vel, w, R, m, kin, kout, dt
+
+
+def case_two_oblique_spin():
+    R = np.array([1.2e-5, 0.9e-5])
+    m = np.array([mass(r) for r in R])
+    kin, kout, dt = spring(R.max(), R.min())
+    v = 15.0
+    S = R.sum()
+    b = 0.45 * S
+    x = math.sqrt(S * S - b * b) + 2 * v * dt * 3
+    pos = np.array([[-x / 2, -b / 2, 0.1e-5], [x / 2, b / 2, -0.1e-5]])
+    vel = np.array([[v, 0.5, -0.3], [-v, -0.2, 0.4]])
+    w = np.array([[1e4, -2e4, 3e4], [-5e3, 4e4, 1e3]])
+    return pos, vel, w, R, m, kin, kout, dt
+
+
+def case_three_unequal():
+    R = np.array([1e-5, 1.5e-5, 0.7e-5])
+    m = np.array([mass(r) for r in R])
+    kin, kout, dt = spring(R.max(), R.min())
+    pos = np.array([[0.0, 0.0, 0.0], [2.45e-5, 0.2e-5, 0.0], [-0.3e-5, 1.68e-5, 0.1e-5]])
+    pos -= (m[:, None] * pos).sum(0) / m.sum()
+    vel = np.array([[3.0, -1.0, 0.5], [-2.0, 1.5, -0.5], [0.5, -4.0, 1.0]])
+    w = np.array([[2e4, 0, -1e4], [0, 3e4, 0], [-1e4, 1e4, 5e4]])
+    return pos, vel, w, R, m, kin, kout, dt
+
+
+def case_cluster(n, seed):
+    rng = np.random.default_rng(seed)
+    pos, R, m = touching_cluster(n, rng)
+    kin, kout, dt = spring(R.max(), R.min())
+    vel = rng.standard_normal((n, 3)) * 2.0
+    vel -= (m[:, None] * vel).sum(0) / m.sum()
+    w = rng.standard_normal((n, 3)) * 1e4
+    return pos, vel, w, R, m, kin, kout, dt
+
+
+CASES = {
+    "two_headon": (case_two_headon, [(2, 1), (201, 10)]),
+    "two_oblique_spin": (case_two_oblique_spin, [(2, 1), (201, 10)]),
+    "three_unequal": (case_three_unequal, [(11, 1), (101, 7)]),
+    "cluster16": (lambda: case_cluster(16, 1601), [(11, 5), (101, 10)]),
+    "cluster64": (lambda: case_cluster(64, 6401), [(101, 25)]),
+}
+
+
+def main():
+    if not oracle.ref_available():
+        oracle.build()
+    if not oracle.ref_available():
+        raise SystemExit("reference harness not built (reference sources absent)")
+    for name, (fn, runs) in CASES.items():
+        pos, vel, w, R, m, kin, kout, dt = fn()
+        mo = np.array([moi(r, mm) for r, mm in zip(R, m)])
+        st = oracle.State(pos.astype(np.float64), vel.astype(np.float64), w.astype(np.float64),
+                          R.astype(np.float64), m.astype(np.float64), mo)
+        for n_steps, skip in runs:
+            out, e_init, ew = oracle.run_reference(st, dt, kin, kout, n_steps, skip)
+            fname = os.path.join(OUT, f"{name}_s{n_steps - 1}_k{skip}.npz")
+            np.savez(fname, pos=st.pos, vel=st.vel, w=st.w, R=st.R, m=st.m, moi=st.moi,
+                     params=np.array([dt, kin, kout]), n_steps=np.array(n_steps),
+                     skip=np.array(skip), out_pos=out["pos"], out_vel=out["vel"],
+                     out_velh=out["velh"], out_w=out["w"], out_wh=out["wh"],
+                     out_acc=out["acc"], out_aacc=out["aacc"], e_init=e_init, e_writes=ew)
+            print(f"{os.path.basename(fname)}: n={st.n} steps={n_steps - 1} writes={len(ew)}")
+
+
+if __name__ == "__main__":
+    main()

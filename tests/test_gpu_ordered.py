@@ -1,0 +1,166 @@
+"""GPU parity of the ordered (bitwise) path, through the C ABI.
+
+Bar: the ordered FP64 path reproduces the reference's single-core arithmetic bit
+for bit in every state array (positions, velocities, half-step velocities,
+spins, accelerations).  Energies agree to rounding (1e-13 relative; the GPU
+sums per block in a fixed order, the reference in one running sum).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import (angmom_scale, bits_equal, golden_names, load_golden,
+                      momentum_scale)
+
+import spacelab_amd as sl
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, os.cpu_count() or 1)
+
+
+def _params_ref(kin, kout, dt, **kw):
+    return sl.preset("reference").replace(kin=float(kin), kout=float(kout), dt=float(dt), **kw)
+
+
+def _oracle_params(oracle, p):
+    return oracle.Params(G=p.G, Ha=p.Ha, h_min=p.h_min, u_s=p.u_s, u_r=p.u_r, kin=p.kin,
+                         kout=p.kout, dt=p.dt, far_gravity=p.far_gravity)
+
+
+def _energy_close(got, ref, m, moi, pos, vel, w, rtol=1e-13):
+    assert abs(got.PE - ref[0]) <= rtol * abs(ref[0]) + 1e-300
+    assert abs(got.KE - ref[1]) <= rtol * abs(ref[1]) + 1e-300
+    ps = momentum_scale(m, vel)
+    Ls = angmom_scale(m, moi, pos, vel, w)
+    assert np.max(np.abs(got.p - ref[2:5])) <= rtol * ps + 1e-300
+    assert np.max(np.abs(got.L - ref[5:8])) <= rtol * Ls + 1e-300
+
+
+def test_device_division_sqrt_correctly_rounded():
+    """Bitwise parity needs IEEE-correct binary64 a/b and sqrt(a) on the device."""
+    rng = np.random.default_rng(11)
+    n = 1 << 20
+    a = np.exp(rng.uniform(-230, 230, n)) * rng.choice([-1.0, 1.0], n)
+    b = np.exp(rng.uniform(-230, 230, n)) * rng.choice([-1.0, 1.0], n)
+    a_pos = np.abs(a)
+    q, s, lg = sl.probe_divsqrt(a_pos, b)
+    assert bits_equal(q, a_pos / b)
+    assert bits_equal(s, np.sqrt(a_pos))
+    # log is only used for the vdW energy; it need not be correctly rounded
+    assert np.max(np.abs(lg - np.log(a_pos)) / np.abs(np.log(a_pos))) < 1e-15
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_golden_bitwise(name):
+    z = load_golden(name)
+    dt, kin, kout = z["params"]
+    g = sl.BallGroup(z["pos"], z["vel"], z["w"], z["R"], z["m"], z["moi"])
+    with sl.Collider(len(z["R"]), _params_ref(kin, kout, dt)) as c:
+        c.upload(g)
+        e0 = c.init_conditions()
+        ew = c.sim_looper(int(z["n_steps"]), int(z["skip"]))
+        out = c.download()
+    for k in ("pos", "vel", "velh", "w", "wh", "acc", "aacc"):
+        assert bits_equal(getattr(out, k), z["out_" + k]), f"{name}: {k}"
+    _energy_close(e0, z["e_init"], z["m"], z["moi"], z["pos"], z["vel"], z["w"])
+    assert len(ew) == len(z["e_writes"])
+    for e, r in zip(ew, z["e_writes"]):
+        _energy_close(e, r, z["m"], z["moi"], out.pos, out.vel, out.w)
+
+
+@pytest.mark.parametrize("far_gravity", [0, 1])
+@pytest.mark.parametrize("v_sigma", [1e-3, 5.0])
+def test_bpca_1000_bitwise_100_steps(oracle_mod, far_gravity, v_sigma):
+    """Aggregate of 1000 spheres (lognormal radii), 100 steps: bitwise vs the oracle."""
+    n = 1000
+    g = sl.generate_bpca(n, seed=0x5EED0000, v_sigma=v_sigma)
+    base = sl.preset("config" if far_gravity else "reference")
+    p = sl.calibrate(base, g.R, g.m, velocity=0.36301555459799423, cor=0.64)
+    with sl.Collider(n, p) as c:
+        c.upload(g)
+        e0 = c.init_conditions()
+        ew = c.step(100, 50)
+        out = c.download()
+    st = oracle_mod.State(g.pos, g.vel, g.w, g.R, g.m, g.moi)
+    s, oe0, oew = oracle_mod.run(st, _oracle_params(oracle_mod, p), 101, 50, mode="gather",
+                                 nthreads=NTHREADS)
+    for k in ("pos", "vel", "velh", "w", "wh", "acc", "aacc"):
+        assert bits_equal(getattr(out, k), getattr(s, k)), k
+    _energy_close(e0, oe0, g.m, g.moi, g.pos, g.vel, g.w, rtol=1e-12)
+    for e, r in zip(ew, oew):
+        _energy_close(e, r, g.m, g.moi, out.pos, out.vel, out.w, rtol=1e-12)
+    assert e0.contacts >= 0
+
+
+def test_energy_10k_steps(oracle_mod):
+    """10k steps of a 200-sphere aggregate: states bitwise, E/p/L within 1e-9."""
+    n = 200
+    g = sl.generate_bpca(n, seed=0x5EED0001, v_sigma=2.0)
+    p = sl.calibrate(sl.preset("config"), g.R, g.m, velocity=0.36301555459799423, cor=0.64)
+    with sl.Collider(n, p) as c:
+        c.upload(g)
+        c.init_conditions()
+        ew = c.step(10000, 1000)
+        out = c.download()
+    st = oracle_mod.State(g.pos, g.vel, g.w, g.R, g.m, g.moi)
+    s, _, oew = oracle_mod.run(st, _oracle_params(oracle_mod, p), 10001, 1000, mode="gather",
+                               nthreads=NTHREADS)
+    for k in ("pos", "vel", "w"):
+        assert bits_equal(getattr(out, k), getattr(s, k)), k
+    assert len(ew) == 10
+    for e, r in zip(ew, oew):
+        assert abs(e.E - (r[0] + r[1])) <= 1e-9 * (abs(r[0]) + r[1])
+        _energy_close(e, r, g.m, g.moi, out.pos, out.vel, out.w, rtol=1e-9)
+
+
+def test_reproducible_run_to_run():
+    n = 500
+    g = sl.generate_bpca(n, seed=99, v_sigma=3.0)
+    p = sl.calibrate(sl.preset("config"), g.R, g.m, velocity=0.36301555459799423, cor=0.64)
+    outs = []
+    for _ in range(2):
+        with sl.Collider(n, p) as c:
+            c.upload(g)
+            c.init_conditions()
+            c.step(50, 0)
+            outs.append(c.download())
+    for k in ("pos", "vel", "w", "acc", "aacc"):
+        assert bits_equal(getattr(outs[0], k), getattr(outs[1], k))
+
+
+def test_errors_are_codes_not_crashes():
+    p = _params_ref(1.0, 0.4, 1e-9)
+    with sl.Collider(8, p) as c:
+        with pytest.raises(sl.SpacelabError) as ei:
+            c.step(1)
+        assert ei.value.code == -3                       # before upload
+        g = sl.generate_bpca(8, seed=1)
+        bad = g.copy()
+        bad.R[3] = 0.0
+        with pytest.raises(sl.SpacelabError) as ei:
+            c.upload(bad)                                # R <= 0
+        assert ei.value.code == -1
+        big = sl.generate_bpca(9, seed=1)
+        with pytest.raises(sl.SpacelabError) as ei:
+            c.upload(big)                                # n > n_max
+        assert ei.value.code == -1
+        c.upload(g)
+        with pytest.raises(sl.SpacelabError) as ei:
+            c.step(1)                                    # no forces yet
+        assert ei.value.code == -3
+
+
+def test_coincident_spheres_flag_nonfinite():
+    """dist = 0 gives NaN forces in the reference too; the library reports it."""
+    R = np.array([1e-5, 1e-5, 1e-5])
+    m = 2.25 * 4 / 3 * np.pi * R ** 3
+    pos = np.array([[0.0, 0, 0], [0.0, 0, 0], [5e-5, 0, 0]])
+    g = sl.BallGroup(pos, np.zeros((3, 3)), np.zeros((3, 3)), R, m, 0.4 * m * R * R)
+    p = _params_ref(1.0, 0.4, 1e-9)
+    with sl.Collider(3, p) as c:
+        c.upload(g)
+        c.init_conditions()
+        e = c.step(1, 1)[0]
+    assert e.nonfinite
