@@ -76,6 +76,34 @@ def run_mode(g, p, order, precision, steps, warmup, prof_steps):
             "e0": e0, "e_end": ew[-1] if ew else None, "state": out}
 
 
+def short_horizon(g, p, modes, steps=100):
+    """Errors of each mode against fp64-ordered (bitwise the reference's arithmetic)
+    after `steps` steps from the same state: the horizon over which trajectories
+    are comparable before contact chaos amplifies rounding differences."""
+    import spacelab_amd as sl
+    n = g.num_particles
+    outs = {}
+    for name, (order, prec) in modes.items():
+        with sl.Collider(n, p.replace(order=order, precision=prec)) as c:
+            c.upload(g)
+            c.init_conditions()
+            e = c.step(steps, steps)[-1]
+            outs[name] = (c.download(), e)
+    ref, eref = outs["fp64-ordered"]
+    res = {}
+    for name, (s, e) in outs.items():
+        if name == "fp64-ordered":
+            continue
+        res[name] = {
+            "steps": steps,
+            "pos_rel": float(np.abs(s.pos - ref.pos).max() / np.abs(ref.pos).max()),
+            "vel_rel": float(np.abs(s.vel - ref.vel).max() / np.abs(ref.vel).max()),
+            "energy_rel": abs(e.E - eref.E) / energy_scale(eref),
+            "angmom_rel": float(np.linalg.norm(e.L - eref.L) / max(np.linalg.norm(eref.L), 1e-300)),
+        }
+    return res
+
+
 def cpu_baseline(g, p, target_s=15.0):
     """The reference's own sim_one_step (oracle/_ref) on a bounded sample of the
     same workload, one host core; falls back to the oracle port if not built."""
@@ -176,6 +204,10 @@ def main():
     if not res:
         raise SystemExit("no mode ran")
     ref = res.get("fp64-ordered")
+    acc = {}
+    if ref is not None and len(res) > 1:
+        log("accuracy: every mode 100 steps from the same state against fp64-ordered")
+        acc = short_horizon(g, p, {k: modes[k] for k in res}, 100)
     modes_out = {}
     for name, r in res.items():
         pr = r["prof"]
@@ -189,14 +221,19 @@ def main():
              "pair_kernel_tflops": flops / (pair_ms * 1e-3) / 1e12 if pair_ms > 0 else None}
         e0, e1 = r["e0"], r["e_end"]
         if e1 is not None:
-            d["energy_drift_rel"] = abs(e1.E - e0.E) / energy_scale(e0)
+            # physical change over the run (contacts dissipate), not a numerical error
+            d["energy_change_rel"] = abs(e1.E - e0.E) / energy_scale(e0)
         if ref is not None and name != "fp64-ordered" and ref["e_end"] is not None and e1 is not None:
-            d["energy_err_vs_fp64_ordered"] = abs(e1.E - ref["e_end"].E) / energy_scale(ref["e_end"])
-            dx = np.abs(r["state"].pos - ref["state"].pos).max()
-            d["pos_err_vs_fp64_ordered"] = float(dx / np.abs(ref["state"].pos).max())
-            Lr = ref["e_end"].L
-            d["angmom_err_vs_fp64_ordered"] = float(np.linalg.norm(e1.L - Lr) /
-                                                    max(np.linalg.norm(Lr), 1e-300))
+            # after the full run trajectories have parted by contact chaos; these are
+            # statistical differences, the short-horizon block holds the accuracy
+            end = {"energy_rel": abs(e1.E - ref["e_end"].E) / energy_scale(ref["e_end"]),
+                   "pos_rel": float(np.abs(r["state"].pos - ref["state"].pos).max()
+                                    / np.abs(ref["state"].pos).max()),
+                   "angmom_rel": float(np.linalg.norm(e1.L - ref["e_end"].L)
+                                       / max(np.linalg.norm(ref["e_end"].L), 1e-300))}
+            d["vs_fp64_ordered_end_of_run"] = end
+        if name in acc:
+            d["vs_fp64_ordered_100_steps"] = acc[name]
         modes_out[name] = d
 
     primary = "fp64-fast" if "fp64-fast" in res else "fp64-ordered"

@@ -153,6 +153,10 @@ int sl_gen_bpca(int n, unsigned long long seed, double r_med, double sigma, doub
  * root are correctly rounded on the device, which bitwise parity relies on. */
 int sl_probe_divsqrt(int n, const double* a, const double* b, double* q, double* s, double* lg);
 
+/* Lane layout of the wavefront register rotation the fast pair kernel relies on:
+ * out[l] (binary32) and out[64+l] (binary64) must be (l+1)%64, out[128+l] (l+31)%64. */
+int sl_probe_rotation(int* out192);
+
 #ifdef __cplusplus
 }
 #endif

@@ -125,6 +125,7 @@ def lib():
     L.sl_gen_bpca.argtypes = [C.c_int, C.c_ulonglong, C.c_double, C.c_double, C.c_double,
                               C.c_double] + [dp] * 6
     L.sl_probe_divsqrt.argtypes = [C.c_int] + [dp] * 5
+    L.sl_probe_rotation.argtypes = [C.POINTER(C.c_int)]
     _lib = L
     return L
 
@@ -309,6 +310,15 @@ class Collider:
         p = _Profile()
         self._check(lib().sl_profile_read(self._ctx, C.byref(p)), "sl_profile_read")
         return {k: getattr(p, k) for k, _ in p._fields_}
+
+
+def probe_rotation() -> np.ndarray:
+    """Lane layout of the wavefront rotation used by the fast pair kernel."""
+    out = (C.c_int * 192)()
+    rc = lib().sl_probe_rotation(out)
+    if rc:
+        raise SpacelabError(rc, "sl_probe_rotation")
+    return np.array(out[:], dtype=np.int64).reshape(3, 64)
 
 
 def probe_divsqrt(a: np.ndarray, b: np.ndarray):

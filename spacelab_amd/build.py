@@ -23,7 +23,8 @@ LIB = os.path.join(LIBDIR, "libspacelab.so")
 
 SOURCES = ["sl_api.hip", "sl_kernels_ordered.hip", "sl_kernels_fast.hip", "sl_util.hip",
            "sl_batch.hip", "sl_mlp.hip", "sl_host.cpp"]
-HEADERS = ["sl_common.hpp", "sl_pair_ref.hpp", "sl_internal.hpp", "sl_fast_pair.hpp"]
+HEADERS = ["sl_common.hpp", "sl_pair_ref.hpp", "sl_internal.hpp", "sl_fast_pair.hpp",
+           "sl_fast_kernels.hpp"]
 
 ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",

@@ -173,6 +173,14 @@ int free_ctx_buffers(sl_ctx* c)
     return SL_OK;
 }
 
+// Number of per-block energy partials the per-ball kernel of the current mode writes.
+int energy_blocks(const sl_ctx* c)
+{
+    return c->p.order == SL_ORDER_FAST ? fast_energy_blocks(c->n) : kick_blocks(c->n);
+}
+
+int energy_blocks_max(int n_max) { return std::max(fast_energy_blocks(n_max), kick_blocks(n_max)); }
+
 // Read back and sum the partials of a write step (fixed order: deterministic).
 int read_energy(sl_ctx* c, int n_pe_blocks, int n_e_blocks, sl_energy* out, bool has_pe)
 {
@@ -201,7 +209,7 @@ int read_energy(sl_ctx* c, int n_pe_blocks, int n_e_blocks, sl_energy* out, bool
 }
 
 // One pair pass (+ per-ball pass) for the current positions.
-int pair_pass(sl_ctx* c, bool write, bool kick, int* n_pe_blocks)
+int pair_pass(sl_ctx* c, bool write, bool kick, bool drift_next, int* n_pe_blocks)
 {
     const Phys P = phys_of(c->p);
     if (write) HIPCHK(c, launch_zero_u64(c->cnt + 1, 1, c->stream));
@@ -219,13 +227,14 @@ int pair_pass(sl_ctx* c, bool write, bool kick, int* n_pe_blocks)
         int rc;
         {
             ProfScope ps(c, 0);
-            rc = fast_pairs(c->fb, c->st, P, c->n, c->p.precision, write, c->cnt, c->stream, &c->err);
+            rc = fast_pairs(c->fb, c->st, P, c->n, c->p.precision, write, c->pe_part, c->cnt, c->stream,
+                            &c->err);
             if (rc) return rc;
         }
         {
             ProfScope ps(c, 1);
-            rc = fast_ball(c->fb, c->st, P, c->n, c->p.precision, write, kick, c->pe_part, c->e_part,
-                           c->cnt, c->nonfinite, c->stream, n_pe_blocks, &c->err);
+            rc = fast_ball(c->fb, c->st, P, c->n, c->p.precision, write, kick, drift_next, c->pe_part,
+                           c->e_part, c->cnt, c->nonfinite, c->stream, n_pe_blocks, &c->err);
             if (rc) return rc;
         }
     }
@@ -296,13 +305,14 @@ int sl_create(sl_ctx** out, int n_max, const sl_params* p)
     };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(SL_E_HIP);
     if (hipMalloc(&c->base, sizeof(double) * (size_t)NFIELD * c->n_pad) != hipSuccess) return fail(SL_E_HIP);
-    const int max_blocks = std::max(ordered_blocks(n_max, nullptr), fast_max_blocks(n_max)) + 8;
+    // ordered_blocks(n) is largest for the smallest block size (8 balls), whatever n <= n_max
+    const int max_blocks = std::max((n_max + 7) / 8, fast_max_blocks(n_max)) + 8;
     if (hipMalloc(&c->pe_part, sizeof(double) * max_blocks) != hipSuccess) return fail(SL_E_HIP);
-    if (hipMalloc(&c->e_part, sizeof(double) * 7 * (kick_blocks(n_max) + 8)) != hipSuccess) return fail(SL_E_HIP);
+    if (hipMalloc(&c->e_part, sizeof(double) * 7 * (energy_blocks_max(n_max) + 8)) != hipSuccess) return fail(SL_E_HIP);
     if (hipMalloc(&c->cnt, sizeof(unsigned long long) * 2) != hipSuccess) return fail(SL_E_HIP);
     if (hipMalloc(&c->nonfinite, sizeof(int)) != hipSuccess) return fail(SL_E_HIP);
     if (hipHostMalloc(&c->h_pe, sizeof(double) * max_blocks, 0) != hipSuccess) return fail(SL_E_HIP);
-    if (hipHostMalloc(&c->h_e, sizeof(double) * 7 * (kick_blocks(n_max) + 8), 0) != hipSuccess) return fail(SL_E_HIP);
+    if (hipHostMalloc(&c->h_e, sizeof(double) * 7 * (energy_blocks_max(n_max) + 8), 0) != hipSuccess) return fail(SL_E_HIP);
     if (hipHostMalloc(&c->h_cnt, sizeof(unsigned long long) * 2, 0) != hipSuccess) return fail(SL_E_HIP);
     if (hipHostMalloc(&c->h_nonfinite, sizeof(int), 0) != hipSuccess) return fail(SL_E_HIP);
     if (fast_alloc(c->fb, n_max, c->n_pad) != 0) return fail(SL_E_HIP);
@@ -321,13 +331,16 @@ int sl_set_params(sl_ctx* c, const sl_params* p)
     if (!c) return SL_E_ARG;
     int rc = check_params(c, p);
     if (rc) return rc;
-    const bool order_changed = p->order != c->p.order || p->precision != c->p.precision;
-    c->p = *p;
-    if (order_changed && c->uploaded) {
-        // the fast path keeps its own state buffers; bring them in line
-        rc = fast_sync_from(c->fb, c->st, c->n, c->stream, &c->err);
+    if (c->uploaded) {
+        // The fast path keeps the current state in either of two buffer sets and
+        // derives per-ball constants from the parameters: bring the canonical set up
+        // to date, then re-derive for the new parameters.
+        rc = fast_sync_to(c->fb, c->st, c->n, c->stream, &c->err);
+        if (rc) return rc;
+        rc = fast_sync_from(c->fb, c->st, phys_of(*p), c->n, c->stream, &c->err);
         if (rc) return rc;
     }
+    c->p = *p;
     return SL_OK;
 }
 
@@ -376,7 +389,7 @@ int sl_upload(sl_ctx* c, int n, const double* pos3, const double* vel3, const do
     c->step = 0;
     c->uploaded = true;
     c->forces_valid = (acc3 != nullptr);
-    int rc = fast_sync_from(c->fb, c->st, n, c->stream, &c->err);
+    int rc = fast_sync_from(c->fb, c->st, phys_of(c->p), n, c->stream, &c->err);
     if (rc) return rc;
     return SL_OK;
 }
@@ -393,10 +406,14 @@ int sl_init_forces(sl_ctx* c, sl_energy* e)
     }
     HIPCHK(c, launch_zero_i32(c->nonfinite, 1, c->stream));
     int npe = 0;
-    int rc = pair_pass(c, true, false, &npe);
+    int rc = pair_pass(c, true, false, false, &npe);
     if (rc) return rc;
-    rc = read_energy(c, npe, kick_blocks(c->n), e, true);
+    rc = read_energy(c, npe, energy_blocks(c), e, true);
     if (rc) return rc;
+    if (c->p.order == SL_ORDER_FAST) {
+        rc = fast_check(c->fb, c->stream, &c->err);
+        if (rc) return rc;
+    }
     c->forces_valid = true;
     return flush_profile(c);
 }
@@ -419,17 +436,22 @@ int sl_step(sl_ctx* c, int n_steps, int write_every, sl_energy* e_out, int e_cap
             HIPCHK(c, launch_drift(c->st, c->n, c->p.dt, c->stream));
         }
         int npe = 0;
-        int rc = pair_pass(c, write, true, &npe);
+        // in the fast path the per-ball kernel also applies the next step's drift
+        int rc = pair_pass(c, write, true, k + 1 < n_steps, &npe);
         if (rc) return rc;
         if (write) {
             sl_energy e;
-            rc = read_energy(c, npe, kick_blocks(c->n), &e, true);
+            rc = read_energy(c, npe, energy_blocks(c), &e, true);
             if (rc) return rc;
             if (e_out && nw < e_cap) e_out[nw] = e;
             nw++;
         }
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (fast) {
+        int rc = fast_check(c->fb, c->stream, &c->err);
+        if (rc) return rc;
+    }
     int rc = flush_profile(c);
     if (rc) return rc;
     return nw;

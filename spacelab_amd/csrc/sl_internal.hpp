@@ -22,33 +22,52 @@ hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
 hipError_t launch_zero_i32(int* p, int n, hipStream_t st);
 
 // ---- fast path (Newton-3 tiles, fp64 or mixed): sl_kernels_fast.hip
+constexpr int FAST_TILE = 64;       // tile edge (balls) of the Newton-3 pair kernel
+constexpr int FAST_TILES_PER_BLOCK = 4;
+constexpr int FAST_MAX_NEAR = 32;   // near partners (touching or within h_min) per ball
+
+// Per-ball static data derived from R and m for the tile kernel.
+struct FastAux {
+    double* q = nullptr;     // sqrt(Ha/6*64) R^3       (binary64 path: C_ij = q_i q_j)
+    double* g = nullptr;     // sqrt(G) m               (G m_i m_j = g_i g_j)
+    float* r32 = nullptr;    // R / L0                  (mixed path, scaled units)
+    float* q32 = nullptr;    // (R / L0)^3
+    float* g32 = nullptr;    // m / M0
+};
+
 struct FastBuffers {
     int n_max = 0, n_pad = 0;
+    int n = 0;                    // system size the tile table was built for
     double* alt_base = nullptr;   // second state set (ping-pong)
     State alt{};
     bool cur_is_alt = false;      // which set holds the current state
-    double* part = nullptr;       // per-tile far-field partial accelerations
-    double* comp = nullptr;       // unused slot kept for layout stability
-    int* nb_count = nullptr;      // near-pair count per ball
-    int* nb_list = nullptr;       // near-pair partners per ball
+    bool drifted = false;         // the next step's drift was already applied
+    FastAux aux{};
+    double L0 = 1, M0 = 1;        // length and mass units of the mixed path
+    int2* tiles = nullptr;        // (I, J) block pairs, I >= J
+    int n_tiles = 0;
+    double* part = nullptr;       // far-field force partials [K][xyz][n_pad]
+    int* nb_count = nullptr;      // near partners found this step, per ball
+    int* nb_list = nullptr;       // [n_pad][FAST_MAX_NEAR]
     int* flags = nullptr;         // [0] near-list overflow
     unsigned long long* near_total = nullptr;
-    float* f32 = nullptr;         // binary32 copies for the mixed path
-    int max_nb = 0;
+    void* scratch_base = nullptr;
 };
 
 int fast_alloc(FastBuffers& fb, int n_max, int n_pad);
 void fast_free(FastBuffers& fb);
 int fast_max_blocks(int n_max);
-int fast_sync_from(FastBuffers& fb, State& canon, int n, hipStream_t st, std::string* err);
+int fast_sync_from(FastBuffers& fb, State& canon, const Phys& P, int n, hipStream_t st, std::string* err);
 int fast_sync_to(FastBuffers& fb, State& canon, int n, hipStream_t st, std::string* err);
 int fast_begin_init(FastBuffers& fb, State& canon, int n, hipStream_t st, std::string* err);
 int fast_drift(FastBuffers& fb, State& canon, int n, double dt, hipStream_t st, std::string* err);
 int fast_pairs(FastBuffers& fb, State& canon, const Phys& P, int n, int precision, int write,
-               unsigned long long* contacts, hipStream_t st, std::string* err);
+               double* pe_part, unsigned long long* contacts, hipStream_t st, std::string* err);
 int fast_ball(FastBuffers& fb, State& canon, const Phys& P, int n, int precision, int write, int kick,
-              double* pe_part, double* e_part, unsigned long long* contacts, int* nonfinite,
-              hipStream_t st, int* n_pe_blocks, std::string* err);
+              int drift_next, double* pe_part, double* e_part, unsigned long long* contacts,
+              int* nonfinite, hipStream_t st, int* n_pe_blocks, std::string* err);
+int fast_check(FastBuffers& fb, hipStream_t st, std::string* err);
+int fast_energy_blocks(int n);   // energy partial blocks written by the fast per-ball kernel
 hipError_t fast_reset_counters(FastBuffers& fb, hipStream_t st);
 long long fast_read_near_total(FastBuffers& fb, hipStream_t st);
 
