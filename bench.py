@@ -172,6 +172,8 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=500)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="run the plain copy kernel on this many MiB first (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); this build does not shard "
@@ -179,6 +181,9 @@ def main():
         sys.exit(2)
 
     import spacelab_amd as sl
+    if a.copy_probe_mb > 0:
+        ms = sl.probe_copy(a.copy_probe_mb << 20, 3)
+        log(f"copy probe: {a.copy_probe_mb} MiB in {ms:.3f} ms ({2 * (a.copy_probe_mb << 20) / ms / 1e6:.0f} GB/s)")
     n = a.n
     g, p = workload(n)
     pairs = n * (n - 1) // 2

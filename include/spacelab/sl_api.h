@@ -157,6 +157,10 @@ int sl_probe_divsqrt(int n, const double* a, const double* b, double* q, double*
  * out[l] (binary32) and out[64+l] (binary64) must be (l+1)%64, out[128+l] (l+31)%64. */
 int sl_probe_rotation(int* out192);
 
+/* A plain device-to-device copy of `bytes` (16 bytes per thread), launched `reps`
+ * times: the reference kernel for scaling HBM traffic counters to bytes. */
+int sl_probe_copy(unsigned long long bytes, int reps, double* ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif

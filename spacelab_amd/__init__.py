@@ -126,6 +126,7 @@ def lib():
                               C.c_double] + [dp] * 6
     L.sl_probe_divsqrt.argtypes = [C.c_int] + [dp] * 5
     L.sl_probe_rotation.argtypes = [C.POINTER(C.c_int)]
+    L.sl_probe_copy.argtypes = [C.c_ulonglong, C.c_int, C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -319,6 +320,15 @@ def probe_rotation() -> np.ndarray:
     if rc:
         raise SpacelabError(rc, "sl_probe_rotation")
     return np.array(out[:], dtype=np.int64).reshape(3, 64)
+
+
+def probe_copy(nbytes: int, reps: int = 1) -> float:
+    """Run the plain copy kernel; returns ms per copy."""
+    ms = C.c_double()
+    rc = lib().sl_probe_copy(int(nbytes), int(reps), C.byref(ms))
+    if rc:
+        raise SpacelabError(rc, "sl_probe_copy")
+    return ms.value
 
 
 def probe_divsqrt(a: np.ndarray, b: np.ndarray):

@@ -1,25 +1,27 @@
 // Device kernels of the fast path (included by sl_kernels_fast.hip).
 //
-//   k_tiles_*  one wavefront per 64x64 tile (I, J) of the pair triangle, I >= J.
-//              Lane l holds ball i = 64 I + l in registers.  The 64 balls of block
-//              J are staged in LDS twice (slots 0..63 and 64..127), so at sweep
-//              step s lane l reads slot l + s -- ball j = (l + s) mod 64 -- at
-//              address base + s*stride, with no index arithmetic.  The reaction
-//              force on j is accumulated in a register that travels with j: after
-//              every step the accumulators rotate one lane down (DPP wave_rol:1),
-//              so lane l always holds the sum of the j it meets next.  Neither LDS
-//              atomics nor cross-lane reductions are needed.  A pair closer than
-//              S + h_min (touching, or in the band where the reference clamps the
-//              vdW gap) contributes nothing here; it is staged in LDS (ballot
-//              compaction) and appended to both balls' near lists in one flush.
-//              Each tile writes its i-side and j-side force sums once, to
-//              part[J][.][i] and part[I][.][j].
+//   k_tiles_*  SPLIT wavefronts per 64x64 tile (I, J) of the pair triangle, I >= J,
+//              each sweeping 64/SPLIT of the tile's diagonals.  Lane l holds ball
+//              i = 64 I + l in registers.  The 64 balls of block J are staged in
+//              LDS twice (slots 0..63 and 64..127), so at sweep step s lane l reads
+//              slot l + s -- ball j = (l + s) mod 64 -- at base + s*stride, with no
+//              index arithmetic.  The reaction force on j is accumulated in a
+//              register that travels with j: after every step the accumulators
+//              rotate one lane down (DPP wave_rol:1), so lane l always holds the
+//              sum of the j it meets next; no LDS atomics or cross-lane reductions.
+//              A pair closer than S + h_min (touching, or in the band where the
+//              reference clamps the vdW gap) contributes nothing to the sweep; it is
+//              recorded as a bit of a per-lane mask, and after the sweep each lane
+//              evaluates its near pairs once with the reference arithmetic and
+//              files the result (partner, force, torque, energy) in both balls'
+//              near-entry lists.  The SPLIT sweeps of a tile are combined in LDS
+//              in a fixed order, and each tile writes its i-side and j-side force
+//              sums once, to part[J][.][i] and part[I][.][j].
 //   k_ball     eight lanes per ball: sum the tile partials (fixed order; Kahan in
-//              the mixed mode), sort the ball's near list in LDS and evaluate its
-//              near pairs with the reference arithmetic, then one lane forms acc
-//              and aacc, applies the closing kick, the write-step energies and
-//              the next step's drift.  It reads the current state set and writes
-//              the other one, because near pairs read neighbours' state.
+//              the mixed mode), sort the ball's near entries by partner index and
+//              sum them in that order, then one lane forms acc and aacc, applies
+//              the closing kick, the write-step energies and the next step's
+//              drift (writing the other state set).
 // Every sum has a fixed order, so results are reproducible run to run.
 #pragma once
 
@@ -33,10 +35,9 @@ namespace sl {
 namespace fastk {
 
 constexpr int TB = FAST_TILE;
-constexpr int WPB = FAST_TILES_PER_BLOCK;
-constexpr int NT = TB * WPB;
+constexpr int NW = 4;                 // wavefronts per block
+constexpr int NT = TB * NW;
 constexpr int MAXNB = FAST_MAX_NEAR;
-constexpr int NSTAGE = 64;           // near pairs staged per wave before a flush
 
 // Lane l receives the value held by lane (l + 1) mod 64 (DPP wave_rol:1).
 __device__ __forceinline__ float rot1(float v)
@@ -80,66 +81,80 @@ __device__ __forceinline__ double block_sum(double v, double* red)
     return r;
 }
 
-__device__ __forceinline__ void near_append(int a, int b, int* nb_count, int* nb_list, int* flags)
-{
-    const int sa = atomicAdd(&nb_count[a], 1);
-    if (sa < MAXNB)
-        nb_list[(size_t)a * MAXNB + sa] = b;
-    else
-        atomicOr(&flags[0], 1);
-}
-
 struct TileArgs {
     const double* px;
     const double* py;
     const double* pz;
     const double* R;
     FastAux aux;
+    State cur;              // full state, read by the near-pair evaluation
+    Phys P;
     int n_pad;
     int n_tiles;
     const int2* tiles;
-    double h_min;
-    double Ha;
     double L0inv2;          // 1 / L0^2 (mixed)
     double Kv, Kg;          // mixed: force = Kv * vdw_sum + Kg * grav_sum
     double* part;
     int* nb_count;
-    int* nb_list;
+    NearEntry* nb_ent;
     int* flags;
     double* pe_part;
 };
 
-// Near hits of the sweep are recorded branch-free as bits of a per-lane mask
-// (bit s: the pair met at step s).  After the sweep each lane turns its bits into
-// (i, j) pairs in the wave's LDS stage (slot from an LDS counter), and the whole
-// wave then appends the staged pairs to the per-ball lists in parallel.  Pairs
-// beyond the stage capacity are appended directly by their lane.
-__device__ void flush_near(const TileArgs& a, unsigned long long mask, int i, int J, int2* stage,
-                           int* stage_count)
+struct NearStateG {
+    const State& s;
+    int A, B;
+    __device__ D3 vA() const { return d3(s.f[VX][A], s.f[VY][A], s.f[VZ][A]); }
+    __device__ D3 vB() const { return d3(s.f[VX][B], s.f[VY][B], s.f[VZ][B]); }
+    __device__ D3 wA() const { return d3(s.f[WX][A], s.f[WY][A], s.f[WZ][A]); }
+    __device__ D3 wB() const { return d3(s.f[WX][B], s.f[WY][B], s.f[WZ][B]); }
+    __device__ double old_dist() const
+    {
+        return norm(d3(s.f[QX][B], s.f[QY][B], s.f[QZ][B]) - d3(s.f[QX][A], s.f[QY][A], s.f[QZ][A]));
+    }
+};
+
+__device__ __forceinline__ void file_entry(const TileArgs& a, int self, int other, int contact,
+                                           D3 F, D3 T, double pe)
+{
+    const int k = atomicAdd(&a.nb_count[self], 1);
+    if (k < MAXNB) {
+        NearEntry e;
+        e.j = other;
+        e.contact = contact;
+        e.fx = F.x; e.fy = F.y; e.fz = F.z;
+        e.tx = T.x; e.ty = T.y; e.tz = T.z;
+        e.pe = pe;
+        a.nb_ent[(size_t)self * MAXNB + k] = e;
+    } else {
+        atomicOr(&a.flags[0], 1);
+    }
+}
+
+// Near pairs of this lane (bit s of mask: the pair met at step s): evaluate each
+// once, in the reference's orientation (A = max index) and arithmetic, and file
+// it with both balls.  The pair energy goes with A only, so it is counted once.
+__device__ void eval_near(const TileArgs& a, unsigned long long mask, int i, int J, int write)
 {
     const int l = threadIdx.x % TB;
-    if (l == 0) *stage_count = 0;
-    __builtin_amdgcn_wave_barrier();
+    const State& c = a.cur;
     while (mask) {
         const int s = __ffsll((long long)mask) - 1;
         mask &= mask - 1;
         const int j = J * TB + ((l + s) & (TB - 1));
-        const int slot = atomicAdd(stage_count, 1);
-        if (slot < NSTAGE) {
-            stage[slot] = make_int2(i, j);
-        } else {
-            near_append(i, j, a.nb_count, a.nb_list, a.flags);
-            near_append(j, i, a.nb_count, a.nb_list, a.flags);
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    int count = *stage_count;
-    count = count < NSTAGE ? count : NSTAGE;
-    if (l < count) {
-        const int2 p = stage[l];
-        near_append(p.x, p.y, a.nb_count, a.nb_list, a.flags);
-        near_append(p.y, p.x, a.nb_count, a.nb_list, a.flags);
+        const int A = i > j ? i : j, B = i > j ? j : i;
+        PairIn in;
+        in.pA = d3(c.f[PX][A], c.f[PY][A], c.f[PZ][A]);
+        in.pB = d3(c.f[PX][B], c.f[PY][B], c.f[PZ][B]);
+        in.RA = c.f[RAD][A];
+        in.RB = c.f[RAD][B];
+        in.mA = c.f[MASS][A];
+        in.mB = c.f[MASS][B];
+        PairOut o;
+        NearStateG ns{c, A, B};
+        pair_ref(a.P, write != 0, in, ns, o);
+        file_entry(a, A, B, o.contact, o.F, o.tA, o.pe);
+        file_entry(a, B, A, o.contact ? 2 : 0, -o.F, o.tB, 0.0);
     }
 }
 
@@ -150,20 +165,36 @@ __device__ __forceinline__ double far_energy(double Ha, double Ri, double Rj, do
     return -Ha / 6 * (two / s + two / t + ::log(s / t));
 }
 
+// Sweep range of part h of SPLIT: off-diagonal tiles sweep s = 0..63; diagonal
+// tiles s = 1..32 (each unordered pair of the block once; s = 32 for the lower
+// half of the lanes only).
+template <int SPLIT>
+__device__ __forceinline__ void sweep_range(bool diag, int h, int& s0, int& s1)
+{
+    if (!diag) {
+        s0 = h * (TB / SPLIT);
+        s1 = (h + 1) * (TB / SPLIT);
+    } else {
+        s0 = 1 + h * ((TB / 2) / SPLIT);
+        s1 = 1 + (h + 1) * ((TB / 2) / SPLIT);
+    }
+}
+
 struct JRec64 {
     double x, y, z, R, q, g;
 };
 
-template <bool WRITE, bool GRAV>
+template <bool WRITE, bool GRAV, int SPLIT>
 __global__ __launch_bounds__(NT) void k_tiles_f64(TileArgs a)
 {
-    __shared__ JRec64 sj[WPB][2 * TB];
-    __shared__ int2 nst[WPB][NSTAGE];
-    __shared__ int nstc[WPB];
+    constexpr int TPB = NW / SPLIT;   // tiles per block
+    __shared__ JRec64 sj[TPB][2 * TB];
+    __shared__ double comb[NW][6][TB];
     __shared__ double red[NT];
 
     const int w = threadIdx.x / TB, l = threadIdx.x % TB;
-    const int t = blockIdx.x * WPB + w;
+    const int tl = w / SPLIT, h = w % SPLIT;
+    const int t = blockIdx.x * TPB + tl;
     const bool live = t < a.n_tiles;
     int I = 0, J = 0;
     if (live) {
@@ -176,7 +207,7 @@ __global__ __launch_bounds__(NT) void k_tiles_f64(TileArgs a)
     const double xi = a.px[i], yi = a.py[i], zi = a.pz[i], Ri = a.R[i];
     const double qi = a.aux.q[i], gi = a.aux.g[i];
     const double Ri4 = 4 * Ri;
-    {
+    if (h == 0) {
         JRec64 r;
         r.x = a.px[j0];
         r.y = a.py[j0];
@@ -184,20 +215,20 @@ __global__ __launch_bounds__(NT) void k_tiles_f64(TileArgs a)
         r.R = a.R[j0];
         r.q = a.aux.q[j0];
         r.g = a.aux.g[j0];
-        sj[w][l] = r;
-        sj[w][l + TB] = r;
+        sj[tl][l] = r;
+        sj[tl][l + TB] = r;
     }
     __syncthreads();
 
-    const double h_min = a.h_min;
+    const double h_min = a.P.h_min;
     const double hmin2 = h_min * h_min;
     const double two_hmin = 2 * h_min;
     double ax = 0, ay = 0, az = 0;      // force on i
     double bx = 0, by = 0, bz = 0;      // reaction on the j this lane meets next
     double pe = 0;
     const bool diag = I == J;
-    const JRec64* rec = &sj[w][l];
-    unsigned long long nmask = 0;   // bit s: near pair met at step s
+    const JRec64* rec = &sj[tl][l];
+    unsigned long long nmask = 0;       // bit s: near pair met at step s
 
     auto pair = [&](int s, bool act) {
         const JRec64& r = rec[s];
@@ -227,7 +258,7 @@ __global__ __launch_bounds__(NT) void k_tiles_f64(TileArgs a)
         by = rot1(__builtin_fma(-f, dy, by));
         bz = rot1(__builtin_fma(-f, dz, bz));
         if (WRITE && use) {
-            double u = far_energy(a.Ha, Ri, Rj, sv, tv);
+            double u = far_energy(a.P.Ha, Ri, Rj, sv, tv);
             if (GRAV) u = u - (gi * r.g) * rinv;
             pe = pe + u;
         }
@@ -235,34 +266,54 @@ __global__ __launch_bounds__(NT) void k_tiles_f64(TileArgs a)
     };
 
     if (live) {
+        int s0, s1;
+        sweep_range<SPLIT>(diag, h, s0, s1);
         if (!diag) {
 #pragma unroll (WRITE ? 1 : 8)
-            for (int s = 0; s < TB; s++) pair(s, true);
+            for (int s = s0; s < s1; s++) pair(s, true);
         } else {
-            // each unordered pair of the block once: offsets 1..31 for every lane,
-            // offset 32 for the lower half only
-#pragma unroll (WRITE ? 1 : 8)
-            for (int s = 1; s < TB / 2; s++) pair(s, true);
-            pair(TB / 2, l < TB / 2);
-            // lane l now holds the reaction sum of slot l + 33: bring slot l home
-            bx = rotk(bx, TB - (TB / 2 + 1));
-            by = rotk(by, TB - (TB / 2 + 1));
-            bz = rotk(bz, TB - (TB / 2 + 1));
+#pragma unroll (WRITE ? 1 : 4)
+            for (int s = s0; s < s1 - 1; s++) pair(s, true);
+            pair(s1 - 1, s1 - 1 < TB / 2 || l < TB / 2);
         }
-        if (__ballot(nmask != 0)) flush_near(a, nmask, i, J, &nst[w][0], &nstc[w]);
+        // lane l holds the reaction sum of slot l + s1: bring slot l home
+        const int back = (TB - (s1 & (TB - 1))) & (TB - 1);
+        if (back) {
+            bx = rotk(bx, back);
+            by = rotk(by, back);
+            bz = rotk(bz, back);
+        }
+        if (__ballot(nmask != 0)) eval_near(a, nmask, i, J, WRITE ? 1 : 0);
+        comb[w][0][l] = ax;
+        comb[w][1][l] = ay;
+        comb[w][2][l] = az;
+        comb[w][3][l] = bx;
+        comb[w][4][l] = by;
+        comb[w][5][l] = bz;
+    }
+    __syncthreads();
+    if (live && h == 0) {
+        // the SPLIT sweeps of this tile, in a fixed order
+        const int w0 = tl * SPLIT;
+        double v[6];
+        for (int k = 0; k < 6; k++) {
+            double acc = comb[w0][k][l];
+            for (int q = 1; q < SPLIT; q++) acc = acc + comb[w0 + q][k][l];
+            v[k] = acc;
+        }
         const size_t np = (size_t)a.n_pad;
         if (diag) {
             // the i side and the j side of the diagonal tile land on the same balls
-            a.part[((size_t)J * 3 + 0) * np + i] = ax + bx;
-            a.part[((size_t)J * 3 + 1) * np + i] = ay + by;
-            a.part[((size_t)J * 3 + 2) * np + i] = az + bz;
+            a.part[((size_t)J * 3 + 0) * np + i] = v[0] + v[3];
+            a.part[((size_t)J * 3 + 1) * np + i] = v[1] + v[4];
+            a.part[((size_t)J * 3 + 2) * np + i] = v[2] + v[5];
         } else {
-            a.part[((size_t)J * 3 + 0) * np + i] = ax;
-            a.part[((size_t)J * 3 + 1) * np + i] = ay;
-            a.part[((size_t)J * 3 + 2) * np + i] = az;
-            a.part[((size_t)I * 3 + 0) * np + j0] = bx;
-            a.part[((size_t)I * 3 + 1) * np + j0] = by;
-            a.part[((size_t)I * 3 + 2) * np + j0] = bz;
+            a.part[((size_t)J * 3 + 0) * np + i] = v[0];
+            a.part[((size_t)J * 3 + 1) * np + i] = v[1];
+            a.part[((size_t)J * 3 + 2) * np + i] = v[2];
+            a.part[((size_t)I * 3 + 0) * np + j0] = v[3];
+            a.part[((size_t)I * 3 + 1) * np + j0] = v[4];
+            a.part[((size_t)I * 3 + 2) * np + j0] = v[5];
         }
     }
     if (WRITE) {
@@ -279,17 +330,18 @@ struct JRec32 {
 // Mixed precision: relative displacement, d^2, the near test and s = d^2 - S^2
 // in binary64; the vdW and gravity factors and the per-tile force sums in
 // binary32 (scaled units; separate sums for vdW and gravity, whose ratio can
-// reach 1e-20); tile sums converted to binary64 forces when written.
-template <bool WRITE, bool GRAV>
+// reach 1e-20); tile sums converted to binary64 forces when combined.
+template <bool WRITE, bool GRAV, int SPLIT>
 __global__ __launch_bounds__(NT) void k_tiles_mixed(TileArgs a)
 {
-    __shared__ JRec32 sj[WPB][2 * TB];
-    __shared__ int2 nst[WPB][NSTAGE];
-    __shared__ int nstc[WPB];
+    constexpr int TPB = NW / SPLIT;
+    __shared__ JRec32 sj[TPB][2 * TB];
+    __shared__ double comb[NW][6][TB];
     __shared__ double red[NT];
 
     const int w = threadIdx.x / TB, l = threadIdx.x % TB;
-    const int t = blockIdx.x * WPB + w;
+    const int tl = w / SPLIT, h = w % SPLIT;
+    const int t = blockIdx.x * TPB + tl;
     const bool live = t < a.n_tiles;
     int I = 0, J = 0;
     if (live) {
@@ -302,7 +354,7 @@ __global__ __launch_bounds__(NT) void k_tiles_mixed(TileArgs a)
     const double xi = a.px[i], yi = a.py[i], zi = a.pz[i], Ri = a.R[i];
     const float r4i = 4.0f * a.aux.r32[i], qi = a.aux.q32[i], gi = a.aux.g32[i];
     const double gi64 = a.aux.g[i];
-    {
+    if (h == 0) {
         JRec32 r;
         r.x = a.px[j0];
         r.y = a.py[j0];
@@ -312,12 +364,12 @@ __global__ __launch_bounds__(NT) void k_tiles_mixed(TileArgs a)
         r.q = a.aux.q32[j0];
         r.g = a.aux.g32[j0];
         r.pad = 0.f;
-        sj[w][l] = r;
-        sj[w][l + TB] = r;
+        sj[tl][l] = r;
+        sj[tl][l + TB] = r;
     }
     __syncthreads();
 
-    const double h_min = a.h_min;
+    const double h_min = a.P.h_min;
     const double hmin2 = h_min * h_min;
     const double two_hmin = 2 * h_min;
     const double L0inv2 = a.L0inv2;
@@ -325,8 +377,8 @@ __global__ __launch_bounds__(NT) void k_tiles_mixed(TileArgs a)
     float wvx = 0.f, wvy = 0.f, wvz = 0.f, wgx = 0.f, wgy = 0.f, wgz = 0.f; // travelling j sums
     double pe = 0;
     const bool diag = I == J;
-    const JRec32* rec = &sj[w][l];
-    unsigned long long nmask = 0;   // bit s: near pair met at step s
+    const JRec32* rec = &sj[tl][l];
+    unsigned long long nmask = 0;
 
     auto pair = [&](int s, bool act) {
         const JRec32& r = rec[s];
@@ -366,7 +418,7 @@ __global__ __launch_bounds__(NT) void k_tiles_mixed(TileArgs a)
         if (WRITE && use) {
             // write-step energy in binary64 (not on the per-step path)
             const double tv = __builtin_fma(4 * Ri, Rj, sv);
-            double u = far_energy(a.Ha, Ri, Rj, sv, tv);
+            double u = far_energy(a.P.Ha, Ri, Rj, sv, tv);
             if (GRAV) u = u - (gi64 * a.aux.g[J * TB + ((l + s) & (TB - 1))]) * rsq_f64(d2);
             pe = pe + u;
         }
@@ -374,43 +426,58 @@ __global__ __launch_bounds__(NT) void k_tiles_mixed(TileArgs a)
     };
 
     if (live) {
+        int s0, s1;
+        sweep_range<SPLIT>(diag, h, s0, s1);
         if (!diag) {
 #pragma unroll (WRITE ? 1 : 8)
-            for (int s = 0; s < TB; s++) pair(s, true);
+            for (int s = s0; s < s1; s++) pair(s, true);
         } else {
-#pragma unroll (WRITE ? 1 : 8)
-            for (int s = 1; s < TB / 2; s++) pair(s, true);
-            pair(TB / 2, l < TB / 2);
-            constexpr int K = TB - (TB / 2 + 1);
-            wvx = rotk(wvx, K); wvy = rotk(wvy, K); wvz = rotk(wvz, K);
+#pragma unroll (WRITE ? 1 : 4)
+            for (int s = s0; s < s1 - 1; s++) pair(s, true);
+            pair(s1 - 1, s1 - 1 < TB / 2 || l < TB / 2);
+        }
+        const int back = (TB - (s1 & (TB - 1))) & (TB - 1);
+        if (back) {
+            wvx = rotk(wvx, back); wvy = rotk(wvy, back); wvz = rotk(wvz, back);
             if (GRAV) {
-                wgx = rotk(wgx, K); wgy = rotk(wgy, K); wgz = rotk(wgz, K);
+                wgx = rotk(wgx, back); wgy = rotk(wgy, back); wgz = rotk(wgz, back);
             }
         }
-        if (__ballot(nmask != 0)) flush_near(a, nmask, i, J, &nst[w][0], &nstc[w]);
-        const size_t np = (size_t)a.n_pad;
+        if (__ballot(nmask != 0)) eval_near(a, nmask, i, J, WRITE ? 1 : 0);
         const double Kv = a.Kv, Kg = a.Kg;
-        double ax = Kv * (double)vx, ay = Kv * (double)vy, az = Kv * (double)vz;
-        double bx = Kv * (double)wvx, by = Kv * (double)wvy, bz = Kv * (double)wvz;
+        double v[6] = {Kv * (double)vx, Kv * (double)vy, Kv * (double)vz,
+                       Kv * (double)wvx, Kv * (double)wvy, Kv * (double)wvz};
         if (GRAV) {
-            ax = ax + Kg * (double)gx;
-            ay = ay + Kg * (double)gy;
-            az = az + Kg * (double)gz;
-            bx = bx + Kg * (double)wgx;
-            by = by + Kg * (double)wgy;
-            bz = bz + Kg * (double)wgz;
+            v[0] = v[0] + Kg * (double)gx;
+            v[1] = v[1] + Kg * (double)gy;
+            v[2] = v[2] + Kg * (double)gz;
+            v[3] = v[3] + Kg * (double)wgx;
+            v[4] = v[4] + Kg * (double)wgy;
+            v[5] = v[5] + Kg * (double)wgz;
         }
+        for (int k = 0; k < 6; k++) comb[w][k][l] = v[k];
+    }
+    __syncthreads();
+    if (live && h == 0) {
+        const int w0 = tl * SPLIT;
+        double v[6];
+        for (int k = 0; k < 6; k++) {
+            double acc = comb[w0][k][l];
+            for (int q = 1; q < SPLIT; q++) acc = acc + comb[w0 + q][k][l];
+            v[k] = acc;
+        }
+        const size_t np = (size_t)a.n_pad;
         if (diag) {
-            a.part[((size_t)J * 3 + 0) * np + i] = ax + bx;
-            a.part[((size_t)J * 3 + 1) * np + i] = ay + by;
-            a.part[((size_t)J * 3 + 2) * np + i] = az + bz;
+            a.part[((size_t)J * 3 + 0) * np + i] = v[0] + v[3];
+            a.part[((size_t)J * 3 + 1) * np + i] = v[1] + v[4];
+            a.part[((size_t)J * 3 + 2) * np + i] = v[2] + v[5];
         } else {
-            a.part[((size_t)J * 3 + 0) * np + i] = ax;
-            a.part[((size_t)J * 3 + 1) * np + i] = ay;
-            a.part[((size_t)J * 3 + 2) * np + i] = az;
-            a.part[((size_t)I * 3 + 0) * np + j0] = bx;
-            a.part[((size_t)I * 3 + 1) * np + j0] = by;
-            a.part[((size_t)I * 3 + 2) * np + j0] = bz;
+            a.part[((size_t)J * 3 + 0) * np + i] = v[0];
+            a.part[((size_t)J * 3 + 1) * np + i] = v[1];
+            a.part[((size_t)J * 3 + 2) * np + i] = v[2];
+            a.part[((size_t)I * 3 + 0) * np + j0] = v[3];
+            a.part[((size_t)I * 3 + 1) * np + j0] = v[4];
+            a.part[((size_t)I * 3 + 2) * np + j0] = v[5];
         }
     }
     if (WRITE) {
@@ -419,26 +486,13 @@ __global__ __launch_bounds__(NT) void k_tiles_mixed(TileArgs a)
     }
 }
 
-struct NearState {
-    const State& s;
-    int A, B;
-    __device__ D3 vA() const { return d3(s.f[VX][A], s.f[VY][A], s.f[VZ][A]); }
-    __device__ D3 vB() const { return d3(s.f[VX][B], s.f[VY][B], s.f[VZ][B]); }
-    __device__ D3 wA() const { return d3(s.f[WX][A], s.f[WY][A], s.f[WZ][A]); }
-    __device__ D3 wB() const { return d3(s.f[WX][B], s.f[WY][B], s.f[WZ][B]); }
-    __device__ double old_dist() const
-    {
-        return norm(d3(s.f[QX][B], s.f[QY][B], s.f[QZ][B]) - d3(s.f[QX][A], s.f[QY][A], s.f[QZ][A]));
-    }
-};
-
 struct BallArgs {
     State cur, nxt;
     Phys P;
     int n, n_pad, nblk;
     const double* part;
     int* nb_count;
-    const int* nb_list;
+    const NearEntry* nb_ent;
     int write, kick, drift_next, compensated;
     double* pe_part;    // this kernel's PE partials (already offset)
     double* e_part;
@@ -462,8 +516,8 @@ __device__ __forceinline__ double lane_group_sum(double v)
 __global__ __launch_bounds__(256) void k_ball(BallArgs a)
 {
     __shared__ double red[256];
-    __shared__ int lst_in[BPB][MAXNB];
-    __shared__ int lst_sorted[BPB][MAXNB];
+    __shared__ int key[BPB][MAXNB];
+    __shared__ int order[BPB][MAXNB];
     const int g = threadIdx.x % LPB;
     const int bl = threadIdx.x / LPB;
     const int i = blockIdx.x * BPB + bl;
@@ -475,27 +529,20 @@ __global__ __launch_bounds__(256) void k_ball(BallArgs a)
     double e[7] = {0, 0, 0, 0, 0, 0, 0};
     unsigned ncontact = 0;
 
-    // the ball's own state, loaded up front: nothing below depends on it until the
-    // kick, and the loads cannot move across the barriers that follow
+    // the ball's own state, loaded up front (the loads cannot cross the barriers below)
     const double hx0 = c.f[HX][ic], hy0 = c.f[HY][ic], hz0 = c.f[HZ][ic];
     const double gx0 = c.f[GX][ic], gy0 = c.f[GY][ic], gz0 = c.f[GZ][ic];
     const double vx0 = c.f[VX][ic], vy0 = c.f[VY][ic], vz0 = c.f[VZ][ic];
     const double wx0 = c.f[WX][ic], wy0 = c.f[WY][ic], wz0 = c.f[WZ][ic];
     const double qx0 = c.f[QX][ic], qy0 = c.f[QY][ic], qz0 = c.f[QZ][ic];
-    const D3 pi = d3(c.f[PX][ic], c.f[PY][ic], c.f[PZ][ic]);
-    const double Ri = c.f[RAD][ic], mi = c.f[MASS][ic], Ii = c.f[MOI][ic];
+    const double px = c.f[PX][ic], py = c.f[PY][ic], pz = c.f[PZ][ic];
+    const double mi = c.f[MASS][ic], Ii = c.f[MOI][ic];
 
-    // near list into LDS, then sorted by partner index (rank = entries below it)
+    // near entries: partner keys into LDS, order by partner index (rank = keys below)
     const int cnt_raw = valid ? a.nb_count[ic] : 0;
     const int cnt = cnt_raw < MAXNB ? cnt_raw : MAXNB;
-    for (int u = g; u < cnt; u += LPB) lst_in[bl][u] = a.nb_list[(size_t)ic * MAXNB + u];
-    __syncthreads();
-    for (int u = g; u < cnt; u += LPB) {
-        const int v = lst_in[bl][u];
-        int below = 0;
-        for (int x = 0; x < cnt; x++) below += lst_in[bl][x] < v;
-        lst_sorted[bl][below] = v;
-    }
+    const NearEntry* ent = a.nb_ent + (size_t)ic * MAXNB;
+    for (int u = g; u < cnt; u += LPB) key[bl][u] = ent[u].j;
 
     // far field: lane g sums the tile partials K = g, g+8, ...
     double fx = 0, fy = 0, fz = 0;
@@ -524,40 +571,31 @@ __global__ __launch_bounds__(256) void k_ball(BallArgs a)
         }
     }
     __syncthreads();
-
-    // near pairs: lane g takes ranks g, g+8, ... with the reference arithmetic
-    D3 nF = d3(0, 0, 0), nT = d3(0, 0, 0);
-    for (int r = g; r < cnt; r += LPB) {
-        const int j = lst_sorted[bl][r];
-        if (j < 0 || j >= a.n || j == i) continue;   // unreachable: entries are distinct partners
-        const bool iA = i > j;
-        const int A = iA ? i : j, B = iA ? j : i;
-        const D3 pj = d3(c.f[PX][j], c.f[PY][j], c.f[PZ][j]);
-        PairIn in;
-        in.pA = iA ? pi : pj;
-        in.pB = iA ? pj : pi;
-        const double Rj = c.f[RAD][j], mj = c.f[MASS][j];
-        in.RA = iA ? Ri : Rj;
-        in.RB = iA ? Rj : Ri;
-        in.mA = iA ? mi : mj;
-        in.mB = iA ? mj : mi;
-        PairOut o;
-        NearState nsr{c, A, B};
-        pair_ref(a.P, a.write && iA, in, nsr, o);
-        if (iA) {
-            nF = nF + o.F;
-            if (o.contact) nT = nT + o.tA;
-            pe = pe + o.pe;
-            ncontact += o.contact;
-        } else {
-            nF = nF - o.F;
-            if (o.contact) nT = nT + o.tB;
-        }
+    for (int u = g; u < cnt; u += LPB) {
+        const int v = key[bl][u];
+        int below = 0;
+        for (int x = 0; x < cnt; x++) below += key[bl][x] < v;
+        order[bl][below] = u;
     }
-    fx = lane_group_sum(fx + nF.x);
-    fy = lane_group_sum(fy + nF.y);
-    fz = lane_group_sum(fz + nF.z);
-    const double tx = lane_group_sum(nT.x), ty = lane_group_sum(nT.y), tz = lane_group_sum(nT.z);
+    __syncthreads();
+
+    // near contributions in partner order: lane g takes ranks g, g+8, ...
+    double nfx = 0, nfy = 0, nfz = 0, ntx = 0, nty = 0, ntz = 0;
+    for (int r = g; r < cnt; r += LPB) {
+        const NearEntry en = ent[order[bl][r]];
+        nfx = nfx + en.fx;
+        nfy = nfy + en.fy;
+        nfz = nfz + en.fz;
+        ntx = ntx + en.tx;
+        nty = nty + en.ty;
+        ntz = ntz + en.tz;
+        pe = pe + en.pe;
+        ncontact += en.contact == 1;
+    }
+    fx = lane_group_sum(fx + nfx);
+    fy = lane_group_sum(fy + nfy);
+    fz = lane_group_sum(fz + nfz);
+    const double tx = lane_group_sum(ntx), ty = lane_group_sum(nty), tz = lane_group_sum(ntz);
     unsigned nnear = 0;
     if (valid && g == 0) {
         nnear = (unsigned)cnt_raw;
@@ -577,7 +615,6 @@ __global__ __launch_bounds__(256) void k_ball(BallArgs a)
             vx = vx0; vy = vy0; vz = vz0;
             wx = wx0; wy = wy0; wz = wz0;
         }
-        const double px = pi.x, py = pi.y, pz = pi.z;
         if (!(isfinite(vx) && isfinite(vy) && isfinite(vz) && isfinite(wx) && isfinite(wy) &&
               isfinite(wz) && isfinite(px) && isfinite(py) && isfinite(pz)))
             atomicOr(a.nonfinite, 1);

@@ -35,6 +35,18 @@ struct FastAux {
     float* g32 = nullptr;    // m / M0
 };
 
+// One evaluated near pair, filed with each of its two balls: the partner, the
+// force and torque on this ball, and (for the reference orientation's A ball only)
+// the pair energy.  contact: 0 none, 1 touching and this ball is A, 2 touching and
+// this ball is B.
+struct NearEntry {
+    int j;
+    int contact;
+    double fx, fy, fz;
+    double tx, ty, tz;
+    double pe;
+};
+
 struct FastBuffers {
     int n_max = 0, n_pad = 0;
     int n = 0;                    // system size the tile table was built for
@@ -47,11 +59,13 @@ struct FastBuffers {
     int2* tiles = nullptr;        // (I, J) block pairs, I >= J
     int n_tiles = 0;
     double* part = nullptr;       // far-field force partials [K][xyz][n_pad]
-    int* nb_count = nullptr;      // near partners found this step, per ball
-    int* nb_list = nullptr;       // [n_pad][FAST_MAX_NEAR]
+    int* nb_count = nullptr;      // near entries filed this step, per ball
+    NearEntry* nb_ent = nullptr;  // [n_pad][FAST_MAX_NEAR]
     int* flags = nullptr;         // [0] near-list overflow
     unsigned long long* near_total = nullptr;
-    void* scratch_base = nullptr;
+    int split = 0;                // wavefronts per 64x64 tile (0: not tuned yet)
+    int split_key = -1;           // (precision, gravity) the choice was tuned for
+    int tile_blocks = 0;          // blocks of the last tile launch (its PE partials)
 };
 
 int fast_alloc(FastBuffers& fb, int n_max, int n_pad);

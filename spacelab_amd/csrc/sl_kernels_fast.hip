@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 namespace sl {
@@ -77,7 +78,8 @@ int fast_max_blocks(int n_max)
 {
     const int nb = nblocks64(n_max);
     const long long tiles = (long long)nb * (nb + 1) / 2;
-    return (int)((tiles + WPB - 1) / WPB) + (n_max + BPB - 1) / BPB;
+    // the finest tile split puts one tile per block
+    return (int)tiles + (n_max + BPB - 1) / BPB;
 }
 
 int fast_energy_blocks(int n) { return (n + BPB - 1) / BPB; }
@@ -97,7 +99,7 @@ int fast_alloc(FastBuffers& fb, int n_max, int n_pad)
     if (hipMalloc(&fb.tiles, sizeof(int2) * (size_t)tiles) != hipSuccess) return -1;
     if (hipMalloc(&fb.part, sizeof(double) * (size_t)nb * 3 * n_pad) != hipSuccess) return -1;
     if (hipMalloc(&fb.nb_count, sizeof(int) * n_pad) != hipSuccess) return -1;
-    if (hipMalloc(&fb.nb_list, sizeof(int) * (size_t)n_pad * MAXNB) != hipSuccess) return -1;
+    if (hipMalloc(&fb.nb_ent, sizeof(NearEntry) * (size_t)n_pad * MAXNB) != hipSuccess) return -1;
     if (hipMalloc(&fb.flags, sizeof(int) * 4) != hipSuccess) return -1;
     if (hipMalloc(&fb.near_total, sizeof(unsigned long long)) != hipSuccess) return -1;
     if (hipMemset(fb.alt_base, 0, sizeof(double) * (size_t)NFIELD * n_pad) != hipSuccess) return -1;
@@ -111,7 +113,7 @@ int fast_alloc(FastBuffers& fb, int n_max, int n_pad)
 void fast_free(FastBuffers& fb)
 {
     void* ptrs[] = {fb.alt_base, fb.aux.q, fb.aux.g, fb.aux.r32, fb.aux.q32, fb.aux.g32, fb.tiles,
-                    fb.part, fb.nb_count, fb.nb_list, fb.flags, fb.near_total};
+                    fb.part, fb.nb_count, fb.nb_ent, fb.flags, fb.near_total};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     fb = FastBuffers{};
@@ -159,6 +161,7 @@ int fast_sync_from(FastBuffers& fb, State& canon, const Phys& P, int n, hipStrea
             return fail("tile table");
         fb.n_tiles = (int)t.size();
         fb.n = n;
+        fb.split = 0;   // re-tune for the new size
     }
     if (launch_zero_i32(fb.nb_count, fb.n_pad, st) != hipSuccess || launch_zero_i32(fb.flags, 4, st) != hipSuccess)
         return fail("clear near lists");
@@ -200,44 +203,131 @@ int fast_drift(FastBuffers& fb, State& canon, int n, double dt, hipStream_t st, 
     return 0;
 }
 
-int fast_pairs(FastBuffers& fb, State& canon, const Phys& P, int n, int precision, int write,
-               double* pe_part, unsigned long long* contacts, hipStream_t st, std::string* err)
+namespace {
+
+template <bool WRITE, bool GRAV, int SPLIT>
+void launch_tiles(const TileArgs& a, int precision, hipStream_t st)
 {
-    (void)contacts;
-    (void)n;
-    State& c = cur_state(fb, canon);
+    constexpr int TPB = NW / SPLIT;
+    const int blocks = (a.n_tiles + TPB - 1) / TPB;
+    if (precision == SL_PREC_MIXED)
+        k_tiles_mixed<WRITE, GRAV, SPLIT><<<blocks, NT, 0, st>>>(a);
+    else
+        k_tiles_f64<WRITE, GRAV, SPLIT><<<blocks, NT, 0, st>>>(a);
+}
+
+template <int SPLIT>
+void launch_tiles_split(const TileArgs& a, int precision, bool write, bool grav, hipStream_t st)
+{
+    if (write)
+        grav ? launch_tiles<true, true, SPLIT>(a, precision, st) : launch_tiles<true, false, SPLIT>(a, precision, st);
+    else
+        grav ? launch_tiles<false, true, SPLIT>(a, precision, st) : launch_tiles<false, false, SPLIT>(a, precision, st);
+}
+
+void launch_tiles_any(const TileArgs& a, int split, int precision, bool write, bool grav, hipStream_t st)
+{
+    if (split == 4)
+        launch_tiles_split<4>(a, precision, write, grav, st);
+    else if (split == 2)
+        launch_tiles_split<2>(a, precision, write, grav, st);
+    else
+        launch_tiles_split<1>(a, precision, write, grav, st);
+}
+
+int tile_blocks_for(int n_tiles, int split)
+{
+    const int tpb = NW / split;
+    return (n_tiles + tpb - 1) / tpb;
+}
+
+TileArgs tile_args(FastBuffers& fb, State& c, const Phys& P, double* pe_part)
+{
     TileArgs a;
     a.px = c.f[PX];
     a.py = c.f[PY];
     a.pz = c.f[PZ];
     a.R = c.f[RAD];
     a.aux = fb.aux;
+    a.cur = c;
+    a.P = P;
     a.n_pad = fb.n_pad;
     a.n_tiles = fb.n_tiles;
     a.tiles = fb.tiles;
-    a.h_min = P.h_min;
-    a.Ha = P.Ha;
     a.L0inv2 = 1.0 / (fb.L0 * fb.L0);
     a.Kv = (P.Ha / 6 * 64) / (fb.L0 * fb.L0);
     a.Kg = P.G * fb.M0 * fb.M0;
     a.part = fb.part;
     a.nb_count = fb.nb_count;
-    a.nb_list = fb.nb_list;
+    a.nb_ent = fb.nb_ent;
     a.flags = fb.flags;
     a.pe_part = pe_part;
-    const int blocks = (fb.n_tiles + WPB - 1) / WPB;
-    const bool g = P.far_gravity != 0;
-    if (precision == SL_PREC_MIXED) {
-        if (write)
-            g ? k_tiles_mixed<true, true><<<blocks, NT, 0, st>>>(a) : k_tiles_mixed<true, false><<<blocks, NT, 0, st>>>(a);
-        else
-            g ? k_tiles_mixed<false, true><<<blocks, NT, 0, st>>>(a) : k_tiles_mixed<false, false><<<blocks, NT, 0, st>>>(a);
-    } else {
-        if (write)
-            g ? k_tiles_f64<true, true><<<blocks, NT, 0, st>>>(a) : k_tiles_f64<true, false><<<blocks, NT, 0, st>>>(a);
-        else
-            g ? k_tiles_f64<false, true><<<blocks, NT, 0, st>>>(a) : k_tiles_f64<false, false><<<blocks, NT, 0, st>>>(a);
+    return a;
+}
+
+// Autotuner: the number of wavefronts sharing one 64x64 tile trades per-wave
+// set-up against occupancy and load balance, which depend on the system size and
+// the machine.  Time each choice on the live state (HIP events on the context's
+// stream, best of three launches) and keep the fastest; the near lists these
+// trial launches fill are cleared again.  SL_TILE_SPLIT=1|2|4 forces a choice.
+int tune_split(FastBuffers& fb, const TileArgs& a, int precision, bool grav, hipStream_t st)
+{
+    const char* env = std::getenv("SL_TILE_SPLIT");
+    if (env) {
+        const int v = std::atoi(env);
+        if (v == 1 || v == 2 || v == 4) return v;
     }
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return 1;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return 1;
+    }
+    int best = 1;
+    float best_ms = 1e30f;
+    const int choices[3] = {1, 2, 4};
+    for (int c : choices) {
+        float ms_min = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {
+            hipEventRecord(e0, st);
+            launch_tiles_any(a, c, precision, false, grav, st);
+            hipEventRecord(e1, st);
+            if (hipEventSynchronize(e1) != hipSuccess) break;
+            float ms = 0;
+            hipEventElapsedTime(&ms, e0, e1);
+            ms_min = ms < ms_min ? ms : ms_min;
+            launch_zero_i32(fb.nb_count, fb.n_pad, st);
+        }
+        if (ms_min < best_ms) {
+            best_ms = ms_min;
+            best = c;
+        }
+    }
+    launch_zero_i32(fb.nb_count, fb.n_pad, st);
+    launch_zero_i32(fb.flags, 4, st);
+    hipStreamSynchronize(st);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return best;
+}
+
+}  // namespace
+
+int fast_pairs(FastBuffers& fb, State& canon, const Phys& P, int n, int precision, int write,
+               double* pe_part, unsigned long long* contacts, hipStream_t st, std::string* err)
+{
+    (void)contacts;
+    (void)n;
+    State& c = cur_state(fb, canon);
+    const TileArgs a = tile_args(fb, c, P, pe_part);
+    const bool g = P.far_gravity != 0;
+    const int key = precision * 2 + (g ? 1 : 0);
+    if (fb.split == 0 || fb.split_key != key) {
+        fb.split = tune_split(fb, a, precision, g, st);
+        fb.split_key = key;
+    }
+    launch_tiles_any(a, fb.split, precision, write != 0, g, st);
+    fb.tile_blocks = tile_blocks_for(fb.n_tiles, fb.split);
     if (hipGetLastError() != hipSuccess) {
         if (err) *err = "fast path: tile kernel launch";
         return SL_E_HIP;
@@ -249,7 +339,7 @@ int fast_ball(FastBuffers& fb, State& canon, const Phys& P, int n, int precision
               int drift_next, double* pe_part, double* e_part, unsigned long long* contacts,
               int* nonfinite, hipStream_t st, int* n_pe_blocks, std::string* err)
 {
-    const int tile_blocks = (fb.n_tiles + WPB - 1) / WPB;
+    const int tile_blocks = fb.tile_blocks;
     BallArgs a;
     a.cur = cur_state(fb, canon);
     a.nxt = nxt_state(fb, canon);
@@ -259,7 +349,7 @@ int fast_ball(FastBuffers& fb, State& canon, const Phys& P, int n, int precision
     a.nblk = nblocks64(n);
     a.part = fb.part;
     a.nb_count = fb.nb_count;
-    a.nb_list = fb.nb_list;
+    a.nb_ent = fb.nb_ent;
     a.write = write;
     a.kick = kick;
     a.drift_next = drift_next;

@@ -69,3 +69,45 @@ extern "C" int sl_probe_divsqrt(int n, const double* a, const double* b, double*
     hipFree(d);
     return rc;
 }
+
+namespace sl {
+
+// Plain streaming copy, 16 bytes per thread: the reference point for scaling the
+// FETCH_SIZE / WRITE_SIZE hardware counters to bytes.
+__global__ void k_copy16(const double2* __restrict__ src, double2* __restrict__ dst, size_t n16)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n16) dst[i] = src[i];
+}
+
+}  // namespace sl
+
+extern "C" int sl_probe_copy(unsigned long long bytes, int reps, double* ms_per_copy)
+{
+    if (bytes < 16 || reps < 1) return SL_E_ARG;
+    const size_t n16 = (size_t)(bytes / 16);
+    double2 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n16 * 16) != hipSuccess) return SL_E_HIP;
+    if (hipMalloc(&b, n16 * 16) != hipSuccess) {
+        hipFree(a);
+        return SL_E_HIP;
+    }
+    int rc = SL_OK;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    if (hipMemset(a, 1, n16 * 16) != hipSuccess) rc = SL_E_HIP;
+    const unsigned blocks = (unsigned)((n16 + 255) / 256);
+    hipEventRecord(e0, 0);
+    for (int r = 0; r < reps && rc == SL_OK; r++) sl::k_copy16<<<blocks, 256>>>(a, b, n16);
+    hipEventRecord(e1, 0);
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) rc = SL_E_HIP;
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (ms_per_copy) *ms_per_copy = ms / reps;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    return rc;
+}
