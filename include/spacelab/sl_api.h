@@ -121,6 +121,51 @@ long long sl_step_count(const sl_ctx* ctx);
 const char* sl_last_error(const sl_ctx* ctx);
 void sl_destroy(sl_ctx* ctx);
 
+/* Options of the SL_ORDER_FAST engine (no reference counterpart).
+ *   SL_OPT_GRAPHS      1 (default): runs of steps between write steps are captured
+ *                      once as HIP graphs and replayed; 0: every kernel launched
+ *                      directly.  Results are identical either way.
+ *   SL_OPT_TILE_SPLIT  wavefronts per 64x64 pair tile: 0 (default) = chosen by the
+ *                      built-in autotuner on first use, or 1, 2, 4.  The split changes
+ *                      the summation order inside a tile, so runs compare bitwise only
+ *                      at equal split.  sl_get_option reports the split in use (0
+ *                      before the first fast step). */
+#define SL_OPT_GRAPHS 1
+#define SL_OPT_TILE_SPLIT 2
+int sl_set_option(sl_ctx* ctx, int option, int value);
+int sl_get_option(sl_ctx* ctx, int option, int* value);
+
+/* ---- batched ensembles (no reference counterpart) --------------------------
+ * Many independent Ball_groups (for instance one collision each) advanced together:
+ * every step is one tile launch and one per-ball launch over all systems, and runs
+ * of steps between write steps are replayed as captured HIP graphs.  Each system
+ * has its own kin, kout and dt; G, Ha, h_min, u_s, u_r, far_gravity and precision are
+ * common.  Only SL_ORDER_FAST is offered.  Arrays are the systems' arrays
+ * concatenated in system order (sum of n_per_sys balls). */
+typedef struct sl_batch sl_batch;
+
+/* max_balls: capacity in balls over all systems (each system is padded to a
+ * multiple of 64 slots internally). */
+int sl_batch_create(sl_batch** out, int max_systems, int max_balls, const sl_params* p);
+/* kin, kout, dt: per system (n_sys values each), or NULL for the value in p. */
+int sl_batch_upload(sl_batch* b, int n_sys, const int* n_per_sys, const double* pos3,
+                    const double* vel3, const double* w3, const double* R, const double* m,
+                    const double* moi, const double* kin, const double* kout, const double* dt);
+/* New per-system kin/kout/dt (NULL keeps the current values); state is kept. */
+int sl_batch_set_system_params(sl_batch* b, const double* kin, const double* kout, const double* dt);
+/* Step-0 pass; e_sys[s] receives system s's energies (may be NULL). */
+int sl_batch_init_forces(sl_batch* b, sl_energy* e_sys);
+/* As sl_step for every system; energies of write step w for system s go to
+ * e[w * n_sys + s] for the first e_cap write steps. */
+int sl_batch_step(sl_batch* b, int n_steps, int write_every, sl_energy* e, int e_cap);
+int sl_batch_download(sl_batch* b, double* pos3, double* vel3, double* w3, double* acc3,
+                      double* aacc3);
+int sl_batch_set_option(sl_batch* b, int option, int value);
+int sl_batch_get_option(sl_batch* b, int option, int* value);
+long long sl_batch_step_count(const sl_batch* b);
+const char* sl_batch_last_error(const sl_batch* b);
+void sl_batch_destroy(sl_batch* b);
+
 /* ---- measurement ---------------------------------------------------------- */
 typedef struct {
     double pair_ms;        /* summed duration of the pair kernel launches (HIP events) */

@@ -12,7 +12,8 @@ The object model mirrors the reference's (ball_group.hpp, ColliderSingleCore.cpp
 * :class:`Collider` owns a device context and provides ``init_conditions()``
   (Ball_group::init_conditions), ``sim_one_step(write_step)`` and
   ``sim_looper(steps, skip)`` (ColliderSingleCore.cpp:78-331), with the energies
-  of write steps returned as :class:`Energy` records (O.PE, O.KE, O.mom, O.ang_mom).
+  of write steps returned as :class:`Energy` records (O.PE, O.KE, O.mom, O.ang_mom);
+* :class:`Batch` steps many independent groups (a collision ensemble) together.
 """
 from __future__ import annotations
 
@@ -27,6 +28,8 @@ LIB_PATH = os.path.join(_PKG, "lib", "libspacelab.so")
 
 PREC_FP64, PREC_MIXED = 0, 1
 ORDER_ORDERED, ORDER_FAST = 0, 1
+OPT_GRAPHS, OPT_TILE_SPLIT = 1, 2
+_OPTS = {"graphs": OPT_GRAPHS, "tile_split": OPT_TILE_SPLIT}
 
 ERRORS = {-1: "SL_E_ARG", -2: "SL_E_HIP", -3: "SL_E_STATE", -4: "SL_E_CAPACITY", -5: "SL_E_NOGPU"}
 
@@ -122,6 +125,23 @@ def lib():
     L.sl_destroy.restype = None
     L.sl_profile_enable.argtypes = [vp, C.c_int, C.c_int]
     L.sl_profile_read.argtypes = [vp, C.POINTER(_Profile)]
+    L.sl_set_option.argtypes = [vp, C.c_int, C.c_int]
+    L.sl_get_option.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    ip = C.POINTER(C.c_int)
+    L.sl_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.POINTER(Params)]
+    L.sl_batch_upload.argtypes = [vp, C.c_int, ip] + [dp] * 9
+    L.sl_batch_set_system_params.argtypes = [vp, dp, dp, dp]
+    L.sl_batch_init_forces.argtypes = [vp, C.POINTER(_Energy)]
+    L.sl_batch_step.argtypes = [vp, C.c_int, C.c_int, C.POINTER(_Energy), C.c_int]
+    L.sl_batch_download.argtypes = [vp] + [dp] * 5
+    L.sl_batch_set_option.argtypes = [vp, C.c_int, C.c_int]
+    L.sl_batch_get_option.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    L.sl_batch_step_count.argtypes = [vp]
+    L.sl_batch_step_count.restype = C.c_longlong
+    L.sl_batch_last_error.argtypes = [vp]
+    L.sl_batch_last_error.restype = C.c_char_p
+    L.sl_batch_destroy.argtypes = [vp]
+    L.sl_batch_destroy.restype = None
     L.sl_gen_bpca.argtypes = [C.c_int, C.c_ulonglong, C.c_double, C.c_double, C.c_double,
                               C.c_double] + [dp] * 6
     L.sl_probe_divsqrt.argtypes = [C.c_int] + [dp] * 5
@@ -304,6 +324,15 @@ class Collider:
     def synchronize(self):
         self._check(lib().sl_synchronize(self._ctx), "sl_synchronize")
 
+    def set_option(self, name: str, value: int):
+        """'graphs' (0/1) or 'tile_split' (0 = autotune, 1, 2, 4) of the fast engine."""
+        self._check(lib().sl_set_option(self._ctx, _OPTS[name], int(value)), "sl_set_option")
+
+    def get_option(self, name: str) -> int:
+        v = C.c_int()
+        self._check(lib().sl_get_option(self._ctx, _OPTS[name], C.byref(v)), "sl_get_option")
+        return v.value
+
     def profile_enable(self, enable: bool = True, reset: bool = True):
         self._check(lib().sl_profile_enable(self._ctx, int(enable), int(reset)), "sl_profile_enable")
 
@@ -311,6 +340,124 @@ class Collider:
         p = _Profile()
         self._check(lib().sl_profile_read(self._ctx, C.byref(p)), "sl_profile_read")
         return {k: getattr(p, k) for k, _ in p._fields_}
+
+
+def _opt_f64(a, n):
+    return None if a is None else _f64(np.broadcast_to(np.asarray(a, dtype=np.float64), (n,)).copy(), (n,))
+
+
+class Batch:
+    """Many independent Ball_groups stepped together by the fast engine.
+
+    Each system has its own kin, kout and dt; the other parameters are common.
+    ``step`` returns, per write step, one :class:`Energy` per system.
+    """
+
+    def __init__(self, max_systems: int, max_balls: int, params: Params):
+        self._b = C.c_void_p()
+        rc = lib().sl_batch_create(C.byref(self._b), int(max_systems), int(max_balls),
+                                   C.byref(params))
+        if rc:
+            raise SpacelabError(rc, "sl_batch_create: no usable gfx950 device" if rc == -5
+                                else "sl_batch_create")
+        self.params = params.copy()
+        self.sizes: list[int] = []
+
+    def _check(self, rc: int, what: str) -> int:
+        if rc < 0:
+            msg = lib().sl_batch_last_error(self._b)
+            raise SpacelabError(rc, f"{what}: {msg.decode() if msg else ''}")
+        return rc
+
+    def close(self):
+        if self._b:
+            lib().sl_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def n_sys(self) -> int:
+        return len(self.sizes)
+
+    def upload(self, groups: list[BallGroup], kin=None, kout=None, dt=None):
+        """kin, kout, dt: one value per system (or a scalar for all; None: the params')."""
+        sizes = [g.num_particles for g in groups]
+        cat = lambda f, shape: np.ascontiguousarray(
+            np.concatenate([_f64(getattr(g, f), shape(g.num_particles)) for g in groups]))
+        pos = cat("pos", lambda n: (n, 3)); vel = cat("vel", lambda n: (n, 3))
+        w = cat("w", lambda n: (n, 3))
+        R = cat("R", lambda n: (n,)); m = cat("m", lambda n: (n,)); moi = cat("moi", lambda n: (n,))
+        ns = len(groups)
+        nps = (C.c_int * ns)(*sizes)
+        ki, ko, d = _opt_f64(kin, ns), _opt_f64(kout, ns), _opt_f64(dt, ns)
+        self._check(lib().sl_batch_upload(self._b, ns, nps, _dp(pos), _dp(vel), _dp(w), _dp(R),
+                                          _dp(m), _dp(moi), _dp(ki), _dp(ko), _dp(d)),
+                    "sl_batch_upload")
+        self.sizes = sizes
+        self._static = (R, m, moi)
+
+    def set_system_params(self, kin=None, kout=None, dt=None):
+        ns = self.n_sys
+        self._check(lib().sl_batch_set_system_params(
+            self._b, _dp(_opt_f64(kin, ns)), _dp(_opt_f64(kout, ns)), _dp(_opt_f64(dt, ns))),
+            "sl_batch_set_system_params")
+
+    def init_conditions(self) -> list[Energy]:
+        buf = (_Energy * self.n_sys)()
+        self._check(lib().sl_batch_init_forces(self._b, buf), "sl_batch_init_forces")
+        return [Energy._from(buf[s]) for s in range(self.n_sys)]
+
+    def step(self, n_steps: int, write_every: int = 0) -> list[list[Energy]]:
+        if write_every > 0:
+            c0 = self.step_count
+            cap = (c0 + n_steps) // write_every - c0 // write_every
+        else:
+            cap = 0
+        ns = self.n_sys
+        buf = (_Energy * max(cap * ns, 1))()
+        nw = self._check(lib().sl_batch_step(self._b, int(n_steps), int(write_every), buf,
+                                             max(cap, 1) if cap else 0), "sl_batch_step")
+        return [[Energy._from(buf[k * ns + s]) for s in range(ns)] for k in range(min(nw, cap))]
+
+    @property
+    def step_count(self) -> int:
+        return int(lib().sl_batch_step_count(self._b))
+
+    def download(self) -> list[BallGroup]:
+        n = sum(self.sizes)
+        pos = np.empty((n, 3)); vel = np.empty((n, 3)); w = np.empty((n, 3))
+        acc = np.empty((n, 3)); aacc = np.empty((n, 3))
+        self._check(lib().sl_batch_download(self._b, _dp(pos), _dp(vel), _dp(w), _dp(acc),
+                                            _dp(aacc)), "sl_batch_download")
+        R, m, moi = self._static
+        out, o = [], 0
+        for k in self.sizes:
+            sl_ = slice(o, o + k)
+            out.append(BallGroup(pos[sl_].copy(), vel[sl_].copy(), w[sl_].copy(), R[sl_].copy(),
+                                 m[sl_].copy(), moi[sl_].copy(), acc[sl_].copy(), aacc[sl_].copy()))
+            o += k
+        return out
+
+    def set_option(self, name: str, value: int):
+        self._check(lib().sl_batch_set_option(self._b, _OPTS[name], int(value)),
+                    "sl_batch_set_option")
+
+    def get_option(self, name: str) -> int:
+        v = C.c_int()
+        self._check(lib().sl_batch_get_option(self._b, _OPTS[name], C.byref(v)),
+                    "sl_batch_get_option")
+        return v.value
 
 
 def probe_rotation() -> np.ndarray:

@@ -46,6 +46,8 @@ struct sl_ctx {
     size_t ev_used = 0;
     struct Mark { int kind; size_t e0, e1; };
     std::vector<Mark> marks;
+    size_t prof_e0 = 0;          // open event of the launch being profiled
+    bool prof_open = false;
     sl_profile acc_prof{};
 };
 
@@ -173,13 +175,26 @@ int free_ctx_buffers(sl_ctx* c)
     return SL_OK;
 }
 
-// Number of per-block energy partials the per-ball kernel of the current mode writes.
-int energy_blocks(const sl_ctx* c)
+// Launch hooks of the fast engine while profiling: an event pair around each launch.
+void hook_before(void* u, int)
 {
-    return c->p.order == SL_ORDER_FAST ? fast_energy_blocks(c->n) : kick_blocks(c->n);
+    sl_ctx* c = static_cast<sl_ctx*>(u);
+    c->prof_open = false;
+    hipEvent_t e = next_event(c);
+    if (e && hipEventRecord(e, c->stream) == hipSuccess) {
+        c->prof_e0 = c->ev_used - 1;
+        c->prof_open = true;
+    }
 }
 
-int energy_blocks_max(int n_max) { return std::max(fast_energy_blocks(n_max), kick_blocks(n_max)); }
+void hook_after(void* u, int kind)
+{
+    sl_ctx* c = static_cast<sl_ctx*>(u);
+    if (!c->prof_open) return;
+    c->prof_open = false;
+    hipEvent_t e = next_event(c);
+    if (e && hipEventRecord(e, c->stream) == hipSuccess) c->marks.push_back({kind, c->prof_e0, c->ev_used - 1});
+}
 
 // Read back and sum the partials of a write step (fixed order: deterministic).
 int read_energy(sl_ctx* c, int n_pe_blocks, int n_e_blocks, sl_energy* out, bool has_pe)
@@ -208,37 +223,29 @@ int read_energy(sl_ctx* c, int n_pe_blocks, int n_e_blocks, sl_energy* out, bool
     return SL_OK;
 }
 
-// One pair pass (+ per-ball pass) for the current positions.
-int pair_pass(sl_ctx* c, bool write, bool kick, bool drift_next, int* n_pe_blocks)
+// One ordered pair pass and per-ball pass for the current positions.
+int pair_pass(sl_ctx* c, bool write, bool kick, int* n_pe_blocks)
 {
     const Phys P = phys_of(c->p);
     if (write) HIPCHK(c, launch_zero_u64(c->cnt + 1, 1, c->stream));
-    if (c->p.order == SL_ORDER_ORDERED) {
-        {
-            ProfScope ps(c, 0);
-            HIPCHK(c, launch_pairs_ordered(c->st, P, c->n, write ? 1 : 0, c->pe_part, c->cnt, c->stream));
-        }
-        *n_pe_blocks = ordered_blocks(c->n, nullptr);
-        {
-            ProfScope ps(c, 1);
-            HIPCHK(c, launch_kick(c->st, c->n, c->p.dt, write ? 1 : 0, kick ? 1 : 0, c->e_part, c->nonfinite, c->stream));
-        }
-    } else {
-        int rc;
-        {
-            ProfScope ps(c, 0);
-            rc = fast_pairs(c->fb, c->st, P, c->n, c->p.precision, write, c->pe_part, c->cnt, c->stream,
-                            &c->err);
-            if (rc) return rc;
-        }
-        {
-            ProfScope ps(c, 1);
-            rc = fast_ball(c->fb, c->st, P, c->n, c->p.precision, write, kick, drift_next, c->pe_part,
-                           c->e_part, c->cnt, c->nonfinite, c->stream, n_pe_blocks, &c->err);
-            if (rc) return rc;
-        }
+    {
+        ProfScope ps(c, 0);
+        HIPCHK(c, launch_pairs_ordered(c->st, P, c->n, write ? 1 : 0, c->pe_part, c->cnt, c->stream));
+    }
+    *n_pe_blocks = ordered_blocks(c->n, nullptr);
+    {
+        ProfScope ps(c, 1);
+        HIPCHK(c, launch_kick(c->st, c->n, c->p.dt, write ? 1 : 0, kick ? 1 : 0, c->e_part, c->nonfinite, c->stream));
     }
     return SL_OK;
+}
+
+// Lay the uploaded balls out as a one-system fast engine.
+int configure_fast(sl_ctx* c)
+{
+    const FastLayout L = fast_layout(1, &c->n);
+    const Phys P = phys_of(c->p);
+    return fast_configure(c->fb, c->st, L, &P, c->p.precision, c->stream, &c->err);
 }
 
 }  // namespace
@@ -306,16 +313,17 @@ int sl_create(sl_ctx** out, int n_max, const sl_params* p)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(SL_E_HIP);
     if (hipMalloc(&c->base, sizeof(double) * (size_t)NFIELD * c->n_pad) != hipSuccess) return fail(SL_E_HIP);
     // ordered_blocks(n) is largest for the smallest block size (8 balls), whatever n <= n_max
-    const int max_blocks = std::max((n_max + 7) / 8, fast_max_blocks(n_max)) + 8;
+    const int max_blocks = (n_max + 7) / 8 + 8;
+    const int e_blocks = kick_blocks(n_max) + 8;
     if (hipMalloc(&c->pe_part, sizeof(double) * max_blocks) != hipSuccess) return fail(SL_E_HIP);
-    if (hipMalloc(&c->e_part, sizeof(double) * 7 * (energy_blocks_max(n_max) + 8)) != hipSuccess) return fail(SL_E_HIP);
+    if (hipMalloc(&c->e_part, sizeof(double) * 7 * e_blocks) != hipSuccess) return fail(SL_E_HIP);
     if (hipMalloc(&c->cnt, sizeof(unsigned long long) * 2) != hipSuccess) return fail(SL_E_HIP);
     if (hipMalloc(&c->nonfinite, sizeof(int)) != hipSuccess) return fail(SL_E_HIP);
     if (hipHostMalloc(&c->h_pe, sizeof(double) * max_blocks, 0) != hipSuccess) return fail(SL_E_HIP);
-    if (hipHostMalloc(&c->h_e, sizeof(double) * 7 * (energy_blocks_max(n_max) + 8), 0) != hipSuccess) return fail(SL_E_HIP);
+    if (hipHostMalloc(&c->h_e, sizeof(double) * 7 * e_blocks, 0) != hipSuccess) return fail(SL_E_HIP);
     if (hipHostMalloc(&c->h_cnt, sizeof(unsigned long long) * 2, 0) != hipSuccess) return fail(SL_E_HIP);
     if (hipHostMalloc(&c->h_nonfinite, sizeof(int), 0) != hipSuccess) return fail(SL_E_HIP);
-    if (fast_alloc(c->fb, n_max, c->n_pad) != 0) return fail(SL_E_HIP);
+    if (fast_alloc(c->fb, c->n_pad, 1) != 0) return fail(SL_E_HIP);
     for (int f = 0; f < NFIELD; f++) c->st.f[f] = c->base + (size_t)f * c->n_pad;
     if (hipMemsetAsync(c->base, 0, sizeof(double) * (size_t)NFIELD * c->n_pad, c->stream) != hipSuccess ||
         hipMemsetAsync(c->cnt, 0, sizeof(unsigned long long) * 2, c->stream) != hipSuccess ||
@@ -335,13 +343,49 @@ int sl_set_params(sl_ctx* c, const sl_params* p)
         // The fast path keeps the current state in either of two buffer sets and
         // derives per-ball constants from the parameters: bring the canonical set up
         // to date, then re-derive for the new parameters.
-        rc = fast_sync_to(c->fb, c->st, c->n, c->stream, &c->err);
+        rc = fast_sync_to(c->fb, c->st, c->stream, &c->err);
         if (rc) return rc;
-        rc = fast_sync_from(c->fb, c->st, phys_of(*p), c->n, c->stream, &c->err);
+        const Phys P = phys_of(*p);
+        rc = fast_set_phys(c->fb, c->st, &P, p->precision, c->stream, &c->err);
         if (rc) return rc;
     }
     c->p = *p;
     return SL_OK;
+}
+
+int sl_set_option(sl_ctx* c, int option, int value)
+{
+    if (!c) return SL_E_ARG;
+    switch (option) {
+    case SL_OPT_GRAPHS:
+        c->fb.use_graphs = value != 0;
+        fast_clear_graphs(c->fb);
+        return SL_OK;
+    case SL_OPT_TILE_SPLIT:
+        if (value != 0 && value != 1 && value != 2 && value != 4)
+            return set_err(c, SL_E_ARG, "tile split must be 0 (autotune), 1, 2 or 4 (got %d)", value);
+        c->fb.forced_split = value;
+        c->fb.split = 0;
+        fast_clear_graphs(c->fb);
+        return SL_OK;
+    default:
+        return set_err(c, SL_E_ARG, "unknown option %d", option);
+    }
+}
+
+int sl_get_option(sl_ctx* c, int option, int* value)
+{
+    if (!c || !value) return SL_E_ARG;
+    switch (option) {
+    case SL_OPT_GRAPHS:
+        *value = c->fb.use_graphs ? 1 : 0;
+        return SL_OK;
+    case SL_OPT_TILE_SPLIT:
+        *value = c->fb.split;
+        return SL_OK;
+    default:
+        return set_err(c, SL_E_ARG, "unknown option %d", option);
+    }
 }
 
 int sl_upload(sl_ctx* c, int n, const double* pos3, const double* vel3, const double* w3,
@@ -389,29 +433,23 @@ int sl_upload(sl_ctx* c, int n, const double* pos3, const double* vel3, const do
     c->step = 0;
     c->uploaded = true;
     c->forces_valid = (acc3 != nullptr);
-    int rc = fast_sync_from(c->fb, c->st, phys_of(c->p), n, c->stream, &c->err);
-    if (rc) return rc;
-    return SL_OK;
+    return configure_fast(c);
 }
 
 int sl_init_forces(sl_ctx* c, sl_energy* e)
 {
     if (!c) return SL_E_ARG;
     if (!c->uploaded) return set_err(c, SL_E_STATE, "init before upload");
-    if (c->p.order == SL_ORDER_ORDERED) {
-        HIPCHK(c, launch_copy_prev(c->st, c->n, c->stream));
-    } else {
-        int rc = fast_begin_init(c->fb, c->st, c->n, c->stream, &c->err);
-        if (rc) return rc;
-    }
-    HIPCHK(c, launch_zero_i32(c->nonfinite, 1, c->stream));
-    int npe = 0;
-    int rc = pair_pass(c, true, false, false, &npe);
-    if (rc) return rc;
-    rc = read_energy(c, npe, energy_blocks(c), e, true);
-    if (rc) return rc;
     if (c->p.order == SL_ORDER_FAST) {
-        rc = fast_check(c->fb, c->stream, &c->err);
+        const int rc = fast_init(c->fb, c->st, e, c->stream, &c->err);
+        if (rc) return rc;
+    } else {
+        HIPCHK(c, launch_copy_prev(c->st, c->n, c->stream));
+        HIPCHK(c, launch_zero_i32(c->nonfinite, 1, c->stream));
+        int npe = 0;
+        int rc = pair_pass(c, true, false, &npe);
+        if (rc) return rc;
+        rc = read_energy(c, npe, kick_blocks(c->n), e, true);
         if (rc) return rc;
     }
     c->forces_valid = true;
@@ -424,35 +462,34 @@ int sl_step(sl_ctx* c, int n_steps, int write_every, sl_energy* e_out, int e_cap
     if (!c->uploaded) return set_err(c, SL_E_STATE, "step before upload");
     if (n_steps < 0) return set_err(c, SL_E_ARG, "n_steps < 0");
     if (!c->forces_valid) return set_err(c, SL_E_STATE, "no forces: call sl_init_forces or upload acc");
+    if (c->p.order == SL_ORDER_FAST) {
+        // the engine launches directly while profiling (events around each kernel),
+        // else replays captured step graphs
+        c->fb.hook = c->prof ? LaunchHook{hook_before, hook_after, c} : LaunchHook{};
+        const int nw = fast_steps(c->fb, c->st, n_steps, &c->step, write_every, e_out, e_cap, c->stream, &c->err);
+        const int rc = flush_profile(c);
+        if (nw < 0) return nw;
+        if (rc) return rc;
+        return nw;
+    }
     int nw = 0;
-    const bool fast = c->p.order == SL_ORDER_FAST;
     for (int k = 0; k < n_steps; k++) {
         c->step++;
         const bool write = write_every > 0 && (c->step % write_every) == 0;
-        if (fast) {
-            int rc = fast_drift(c->fb, c->st, c->n, c->p.dt, c->stream, &c->err);
-            if (rc) return rc;
-        } else {
-            HIPCHK(c, launch_drift(c->st, c->n, c->p.dt, c->stream));
-        }
+        HIPCHK(c, launch_drift(c->st, c->n, c->p.dt, c->stream));
         int npe = 0;
-        // in the fast path the per-ball kernel also applies the next step's drift
-        int rc = pair_pass(c, write, true, k + 1 < n_steps, &npe);
+        int rc = pair_pass(c, write, true, &npe);
         if (rc) return rc;
         if (write) {
             sl_energy e;
-            rc = read_energy(c, npe, energy_blocks(c), &e, true);
+            rc = read_energy(c, npe, kick_blocks(c->n), &e, true);
             if (rc) return rc;
             if (e_out && nw < e_cap) e_out[nw] = e;
             nw++;
         }
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (fast) {
-        int rc = fast_check(c->fb, c->stream, &c->err);
-        if (rc) return rc;
-    }
-    int rc = flush_profile(c);
+    const int rc = flush_profile(c);
     if (rc) return rc;
     return nw;
 }
@@ -461,10 +498,8 @@ int sl_download(sl_ctx* c, double* pos3, double* vel3, double* w3, double* acc3,
 {
     if (!c) return SL_E_ARG;
     if (!c->uploaded) return set_err(c, SL_E_STATE, "download before upload");
-    if (c->p.order == SL_ORDER_FAST) {
-        int rc = fast_sync_to(c->fb, c->st, c->n, c->stream, &c->err);
-        if (rc) return rc;
-    }
+    const int rc = fast_sync_to(c->fb, c->st, c->stream, &c->err);
+    if (rc) return rc;
     const size_t np = (size_t)c->n_pad;
     std::vector<double> h((size_t)NFIELD * np);
     HIPCHK(c, hipMemcpyAsync(h.data(), c->base, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
@@ -489,10 +524,8 @@ int sl_download_half(sl_ctx* c, double* velh3, double* wh3)
 {
     if (!c) return SL_E_ARG;
     if (!c->uploaded) return set_err(c, SL_E_STATE, "download before upload");
-    if (c->p.order == SL_ORDER_FAST) {
-        int rc = fast_sync_to(c->fb, c->st, c->n, c->stream, &c->err);
-        if (rc) return rc;
-    }
+    const int rc = fast_sync_to(c->fb, c->st, c->stream, &c->err);
+    if (rc) return rc;
     const size_t np = (size_t)c->n_pad;
     std::vector<double> h(6 * np);
     HIPCHK(c, hipMemcpyAsync(h.data(), c->st.f[HX], sizeof(double) * 6 * np, hipMemcpyDeviceToHost, c->stream));
@@ -550,7 +583,10 @@ int sl_profile_read(sl_ctx* c, sl_profile* out)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *out = c->acc_prof;
     out->contacts = (long long)c->h_cnt[0];
-    out->near_pairs = fast_read_near_total(c->fb, c->stream);
+    long long fc = 0, fn = 0;
+    if (fast_read_counters(c->fb, c->stream, &fc, &fn) != 0) return set_err(c, SL_E_HIP, "reading counters failed");
+    out->contacts += fc;
+    out->near_pairs = fn;
     return SL_OK;
 }
 

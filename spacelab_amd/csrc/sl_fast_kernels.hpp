@@ -88,17 +88,17 @@ struct TileArgs {
     const double* R;
     FastAux aux;
     State cur;              // full state, read by the near-pair evaluation
-    Phys P;
+    const Phys* sysP;       // per system
     int n_pad;
     int n_tiles;
-    const int2* tiles;
+    const int4* tiles;      // {I, J, system, first block of the system}
     double L0inv2;          // 1 / L0^2 (mixed)
     double Kv, Kg;          // mixed: force = Kv * vdw_sum + Kg * grav_sum
     double* part;
     int* nb_count;
     NearEntry* nb_ent;
     int* flags;
-    double* pe_part;
+    double* pe_tile;
 };
 
 struct NearStateG {
@@ -134,7 +134,7 @@ __device__ __forceinline__ void file_entry(const TileArgs& a, int self, int othe
 // Near pairs of this lane (bit s of mask: the pair met at step s): evaluate each
 // once, in the reference's orientation (A = max index) and arithmetic, and file
 // it with both balls.  The pair energy goes with A only, so it is counted once.
-__device__ void eval_near(const TileArgs& a, unsigned long long mask, int i, int J, int write)
+__device__ void eval_near(const TileArgs& a, const Phys& P, unsigned long long mask, int i, int J, int write)
 {
     const int l = threadIdx.x % TB;
     const State& c = a.cur;
@@ -152,10 +152,17 @@ __device__ void eval_near(const TileArgs& a, unsigned long long mask, int i, int
         in.mB = c.f[MASS][B];
         PairOut o;
         NearStateG ns{c, A, B};
-        pair_ref(a.P, write != 0, in, ns, o);
+        pair_ref(P, write != 0, in, ns, o);
         file_entry(a, A, B, o.contact, o.F, o.tA, o.pe);
         file_entry(a, B, A, o.contact ? 2 : 0, -o.F, o.tB, 0.0);
     }
+}
+
+// Sum over the 64 lanes of a wavefront as a fixed tree (deterministic).
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int m = 1; m < TB; m <<= 1) v = v + __shfl_xor(v, m, TB);
+    return v;
 }
 
 // vdW energy of a far pair from s = d^2 - S^2, t = d^2 - (Ri-Rj)^2.
@@ -190,18 +197,21 @@ __global__ __launch_bounds__(NT) void k_tiles_f64(TileArgs a)
     constexpr int TPB = NW / SPLIT;   // tiles per block
     __shared__ JRec64 sj[TPB][2 * TB];
     __shared__ double comb[NW][6][TB];
-    __shared__ double red[NT];
+    __shared__ double pe_w[NW];
 
     const int w = threadIdx.x / TB, l = threadIdx.x % TB;
     const int tl = w / SPLIT, h = w % SPLIT;
     const int t = blockIdx.x * TPB + tl;
     const bool live = t < a.n_tiles;
-    int I = 0, J = 0;
+    int I = 0, J = 0, sys = 0, blk0 = 0;
     if (live) {
-        const int2 ij = a.tiles[t];
-        I = ij.x;
-        J = ij.y;
+        const int4 tt = a.tiles[t];
+        I = tt.x;
+        J = tt.y;
+        sys = tt.z;
+        blk0 = tt.w;
     }
+    const Phys& P = a.sysP[sys];
     const int i = I * TB + l;
     const int j0 = J * TB + l;
     const double xi = a.px[i], yi = a.py[i], zi = a.pz[i], Ri = a.R[i];
@@ -220,7 +230,7 @@ __global__ __launch_bounds__(NT) void k_tiles_f64(TileArgs a)
     }
     __syncthreads();
 
-    const double h_min = a.P.h_min;
+    const double h_min = P.h_min;
     const double hmin2 = h_min * h_min;
     const double two_hmin = 2 * h_min;
     double ax = 0, ay = 0, az = 0;      // force on i
@@ -258,7 +268,7 @@ __global__ __launch_bounds__(NT) void k_tiles_f64(TileArgs a)
         by = rot1(__builtin_fma(-f, dy, by));
         bz = rot1(__builtin_fma(-f, dz, bz));
         if (WRITE && use) {
-            double u = far_energy(a.P.Ha, Ri, Rj, sv, tv);
+            double u = far_energy(P.Ha, Ri, Rj, sv, tv);
             if (GRAV) u = u - (gi * r.g) * rinv;
             pe = pe + u;
         }
@@ -283,7 +293,11 @@ __global__ __launch_bounds__(NT) void k_tiles_f64(TileArgs a)
             by = rotk(by, back);
             bz = rotk(bz, back);
         }
-        if (__ballot(nmask != 0)) eval_near(a, nmask, i, J, WRITE ? 1 : 0);
+        if (__ballot(nmask != 0)) eval_near(a, P, nmask, i, J, WRITE ? 1 : 0);
+        if (WRITE) {
+            const double ws = wave_sum(pe);
+            if (l == 0) pe_w[w] = ws;
+        }
         comb[w][0][l] = ax;
         comb[w][1][l] = ay;
         comb[w][2][l] = az;
@@ -304,21 +318,22 @@ __global__ __launch_bounds__(NT) void k_tiles_f64(TileArgs a)
         const size_t np = (size_t)a.n_pad;
         if (diag) {
             // the i side and the j side of the diagonal tile land on the same balls
-            a.part[((size_t)J * 3 + 0) * np + i] = v[0] + v[3];
-            a.part[((size_t)J * 3 + 1) * np + i] = v[1] + v[4];
-            a.part[((size_t)J * 3 + 2) * np + i] = v[2] + v[5];
+            a.part[((size_t)(J - blk0) * 3 + 0) * np + i] = v[0] + v[3];
+            a.part[((size_t)(J - blk0) * 3 + 1) * np + i] = v[1] + v[4];
+            a.part[((size_t)(J - blk0) * 3 + 2) * np + i] = v[2] + v[5];
         } else {
-            a.part[((size_t)J * 3 + 0) * np + i] = v[0];
-            a.part[((size_t)J * 3 + 1) * np + i] = v[1];
-            a.part[((size_t)J * 3 + 2) * np + i] = v[2];
-            a.part[((size_t)I * 3 + 0) * np + j0] = v[3];
-            a.part[((size_t)I * 3 + 1) * np + j0] = v[4];
-            a.part[((size_t)I * 3 + 2) * np + j0] = v[5];
+            a.part[((size_t)(J - blk0) * 3 + 0) * np + i] = v[0];
+            a.part[((size_t)(J - blk0) * 3 + 1) * np + i] = v[1];
+            a.part[((size_t)(J - blk0) * 3 + 2) * np + i] = v[2];
+            a.part[((size_t)(I - blk0) * 3 + 0) * np + j0] = v[3];
+            a.part[((size_t)(I - blk0) * 3 + 1) * np + j0] = v[4];
+            a.part[((size_t)(I - blk0) * 3 + 2) * np + j0] = v[5];
         }
     }
-    if (WRITE) {
-        const double tot = block_sum<NT>(pe, red);
-        if (threadIdx.x == 0) a.pe_part[blockIdx.x] = tot;
+    if (WRITE && live && h == 0 && l == 0) {
+        double pt = pe_w[tl * SPLIT];
+        for (int q = 1; q < SPLIT; q++) pt = pt + pe_w[tl * SPLIT + q];
+        a.pe_tile[t] = pt;
     }
 }
 
@@ -337,18 +352,21 @@ __global__ __launch_bounds__(NT) void k_tiles_mixed(TileArgs a)
     constexpr int TPB = NW / SPLIT;
     __shared__ JRec32 sj[TPB][2 * TB];
     __shared__ double comb[NW][6][TB];
-    __shared__ double red[NT];
+    __shared__ double pe_w[NW];
 
     const int w = threadIdx.x / TB, l = threadIdx.x % TB;
     const int tl = w / SPLIT, h = w % SPLIT;
     const int t = blockIdx.x * TPB + tl;
     const bool live = t < a.n_tiles;
-    int I = 0, J = 0;
+    int I = 0, J = 0, sys = 0, blk0 = 0;
     if (live) {
-        const int2 ij = a.tiles[t];
-        I = ij.x;
-        J = ij.y;
+        const int4 tt = a.tiles[t];
+        I = tt.x;
+        J = tt.y;
+        sys = tt.z;
+        blk0 = tt.w;
     }
+    const Phys& P = a.sysP[sys];
     const int i = I * TB + l;
     const int j0 = J * TB + l;
     const double xi = a.px[i], yi = a.py[i], zi = a.pz[i], Ri = a.R[i];
@@ -369,7 +387,7 @@ __global__ __launch_bounds__(NT) void k_tiles_mixed(TileArgs a)
     }
     __syncthreads();
 
-    const double h_min = a.P.h_min;
+    const double h_min = P.h_min;
     const double hmin2 = h_min * h_min;
     const double two_hmin = 2 * h_min;
     const double L0inv2 = a.L0inv2;
@@ -418,7 +436,7 @@ __global__ __launch_bounds__(NT) void k_tiles_mixed(TileArgs a)
         if (WRITE && use) {
             // write-step energy in binary64 (not on the per-step path)
             const double tv = __builtin_fma(4 * Ri, Rj, sv);
-            double u = far_energy(a.P.Ha, Ri, Rj, sv, tv);
+            double u = far_energy(P.Ha, Ri, Rj, sv, tv);
             if (GRAV) u = u - (gi64 * a.aux.g[J * TB + ((l + s) & (TB - 1))]) * rsq_f64(d2);
             pe = pe + u;
         }
@@ -443,7 +461,11 @@ __global__ __launch_bounds__(NT) void k_tiles_mixed(TileArgs a)
                 wgx = rotk(wgx, back); wgy = rotk(wgy, back); wgz = rotk(wgz, back);
             }
         }
-        if (__ballot(nmask != 0)) eval_near(a, nmask, i, J, WRITE ? 1 : 0);
+        if (__ballot(nmask != 0)) eval_near(a, P, nmask, i, J, WRITE ? 1 : 0);
+        if (WRITE) {
+            const double ws = wave_sum(pe);
+            if (l == 0) pe_w[w] = ws;
+        }
         const double Kv = a.Kv, Kg = a.Kg;
         double v[6] = {Kv * (double)vx, Kv * (double)vy, Kv * (double)vz,
                        Kv * (double)wvx, Kv * (double)wvy, Kv * (double)wvz};
@@ -468,37 +490,40 @@ __global__ __launch_bounds__(NT) void k_tiles_mixed(TileArgs a)
         }
         const size_t np = (size_t)a.n_pad;
         if (diag) {
-            a.part[((size_t)J * 3 + 0) * np + i] = v[0] + v[3];
-            a.part[((size_t)J * 3 + 1) * np + i] = v[1] + v[4];
-            a.part[((size_t)J * 3 + 2) * np + i] = v[2] + v[5];
+            a.part[((size_t)(J - blk0) * 3 + 0) * np + i] = v[0] + v[3];
+            a.part[((size_t)(J - blk0) * 3 + 1) * np + i] = v[1] + v[4];
+            a.part[((size_t)(J - blk0) * 3 + 2) * np + i] = v[2] + v[5];
         } else {
-            a.part[((size_t)J * 3 + 0) * np + i] = v[0];
-            a.part[((size_t)J * 3 + 1) * np + i] = v[1];
-            a.part[((size_t)J * 3 + 2) * np + i] = v[2];
-            a.part[((size_t)I * 3 + 0) * np + j0] = v[3];
-            a.part[((size_t)I * 3 + 1) * np + j0] = v[4];
-            a.part[((size_t)I * 3 + 2) * np + j0] = v[5];
+            a.part[((size_t)(J - blk0) * 3 + 0) * np + i] = v[0];
+            a.part[((size_t)(J - blk0) * 3 + 1) * np + i] = v[1];
+            a.part[((size_t)(J - blk0) * 3 + 2) * np + i] = v[2];
+            a.part[((size_t)(I - blk0) * 3 + 0) * np + j0] = v[3];
+            a.part[((size_t)(I - blk0) * 3 + 1) * np + j0] = v[4];
+            a.part[((size_t)(I - blk0) * 3 + 2) * np + j0] = v[5];
         }
     }
-    if (WRITE) {
-        const double tot = block_sum<NT>(pe, red);
-        if (threadIdx.x == 0) a.pe_part[blockIdx.x] = tot;
+    if (WRITE && live && h == 0 && l == 0) {
+        double pt = pe_w[tl * SPLIT];
+        for (int q = 1; q < SPLIT; q++) pt = pt + pe_w[tl * SPLIT + q];
+        a.pe_tile[t] = pt;
     }
 }
 
 struct BallArgs {
     State cur, nxt;
-    Phys P;
-    int n, n_pad, nblk;
+    const Phys* sysP;
+    const int* blk_sys;
+    const int* sys_off;
+    const int* sys_n;
+    const int* sys_nb;
+    int n_used, n_pad;
     const double* part;
     int* nb_count;
     const NearEntry* nb_ent;
     int write, kick, drift_next, compensated;
-    double* pe_part;    // this kernel's PE partials (already offset)
-    double* e_part;
-    unsigned long long* contacts;
-    unsigned long long* near_total;
-    int* nonfinite;
+    double* e_blk;      // FAST_E_STRIDE doubles per 32-ball block (write steps)
+    int* flags;         // [1]: a non-finite state appeared
+    unsigned long long* counters;
 };
 
 constexpr int LPB = 8;               // lanes per ball in k_ball
@@ -521,12 +546,13 @@ __global__ __launch_bounds__(256) void k_ball(BallArgs a)
     const int g = threadIdx.x % LPB;
     const int bl = threadIdx.x / LPB;
     const int i = blockIdx.x * BPB + bl;
-    const bool valid = i < a.n;
+    const int sys = a.blk_sys[(i < a.n_used ? i : 0) / TB];
+    const bool valid = i < a.n_used && (i - a.sys_off[sys]) < a.sys_n[sys];
     const int ic = valid ? i : 0;
+    const int nblk = a.sys_nb[sys];
     const State& c = a.cur;
     const size_t np = (size_t)a.n_pad;
-    double pe = 0;
-    double e[7] = {0, 0, 0, 0, 0, 0, 0};
+    double e[FAST_E_STRIDE] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // PE KE p L contacts nonfinite
     unsigned ncontact = 0;
 
     // the ball's own state, loaded up front (the loads cannot cross the barriers below)
@@ -544,13 +570,13 @@ __global__ __launch_bounds__(256) void k_ball(BallArgs a)
     const NearEntry* ent = a.nb_ent + (size_t)ic * MAXNB;
     for (int u = g; u < cnt; u += LPB) key[bl][u] = ent[u].j;
 
-    // far field: lane g sums the tile partials K = g, g+8, ...
+    // far field: lane g sums the tile partials K = g, g+8, ... of the ball's system
     double fx = 0, fy = 0, fz = 0;
     if (valid) {
         if (a.compensated) {
             double cx = 0, cy = 0, cz = 0;   // Kahan compensation
 #pragma unroll 4
-            for (int K = g; K < a.nblk; K += LPB) {
+            for (int K = g; K < nblk; K += LPB) {
                 const double* p = a.part + (size_t)K * 3 * np + ic;
                 double y, t;
                 y = p[0] - cx; t = fx + y; cx = (t - fx) - y; fx = t;
@@ -562,7 +588,7 @@ __global__ __launch_bounds__(256) void k_ball(BallArgs a)
             fz = fz - cz;
         } else {
 #pragma unroll 4
-            for (int K = g; K < a.nblk; K += LPB) {
+            for (int K = g; K < nblk; K += LPB) {
                 const double* p = a.part + (size_t)K * 3 * np + ic;
                 fx = fx + p[0];
                 fy = fy + p[np];
@@ -580,7 +606,7 @@ __global__ __launch_bounds__(256) void k_ball(BallArgs a)
     __syncthreads();
 
     // near contributions in partner order: lane g takes ranks g, g+8, ...
-    double nfx = 0, nfy = 0, nfz = 0, ntx = 0, nty = 0, ntz = 0;
+    double nfx = 0, nfy = 0, nfz = 0, ntx = 0, nty = 0, ntz = 0, pe = 0;
     for (int r = g; r < cnt; r += LPB) {
         const NearEntry en = ent[order[bl][r]];
         nfx = nfx + en.fx;
@@ -596,13 +622,15 @@ __global__ __launch_bounds__(256) void k_ball(BallArgs a)
     fy = lane_group_sum(fy + nfy);
     fz = lane_group_sum(fz + nfz);
     const double tx = lane_group_sum(ntx), ty = lane_group_sum(nty), tz = lane_group_sum(ntz);
+    pe = lane_group_sum(pe);
+    ncontact = (unsigned)lane_group_sum((double)ncontact);
     unsigned nnear = 0;
     if (valid && g == 0) {
         nnear = (unsigned)cnt_raw;
         a.nb_count[i] = 0;   // ready for the next step's tile pass
         const double ax = fx / mi, ay = fy / mi, az = fz / mi;
         const double bx = tx / Ii, by = ty / Ii, bz = tz / Ii;
-        const double dt = a.P.dt;
+        const double dt = a.sysP[sys].dt;
         double vx, vy, vz, wx, wy, wz;
         if (a.kick) {
             vx = hx0 + .5 * ax * dt;
@@ -615,16 +643,19 @@ __global__ __launch_bounds__(256) void k_ball(BallArgs a)
             vx = vx0; vy = vy0; vz = vz0;
             wx = wx0; wy = wy0; wz = wz0;
         }
-        if (!(isfinite(vx) && isfinite(vy) && isfinite(vz) && isfinite(wx) && isfinite(wy) &&
-              isfinite(wz) && isfinite(px) && isfinite(py) && isfinite(pz)))
-            atomicOr(a.nonfinite, 1);
+        const bool bad = !(isfinite(vx) && isfinite(vy) && isfinite(vz) && isfinite(wx) && isfinite(wy) &&
+                           isfinite(wz) && isfinite(px) && isfinite(py) && isfinite(pz));
+        if (bad) atomicOr(&a.flags[1], 1);
         if (a.write) {
             const D3 v = d3(vx, vy, vz), w = d3(wx, wy, wz), p = d3(px, py, pz);
-            e[0] = .5 * mi * dot(v, v) + .5 * Ii * dot(w, w);
+            e[0] = pe;
+            e[1] = .5 * mi * dot(v, v) + .5 * Ii * dot(w, w);
             const D3 mv = v * mi;
-            e[1] = mv.x; e[2] = mv.y; e[3] = mv.z;
+            e[2] = mv.x; e[3] = mv.y; e[4] = mv.z;
             const D3 L = cross(p, v) * mi + w * Ii;
-            e[4] = L.x; e[5] = L.y; e[6] = L.z;
+            e[5] = L.x; e[6] = L.y; e[7] = L.z;
+            e[8] = (double)ncontact;
+            e[9] = bad ? 1.0 : 0.0;
         }
         const State& o = a.nxt;
         o.f[VX][i] = vx; o.f[VY][i] = vy; o.f[VZ][i] = vz;
@@ -649,30 +680,53 @@ __global__ __launch_bounds__(256) void k_ball(BallArgs a)
         }
     }
     if (a.write) {
-        const double tot = block_sum<256>(pe, red);
-        if (threadIdx.x == 0) a.pe_part[blockIdx.x] = tot;
-        for (int k = 0; k < 7; k++) {
+        // a 32-ball block lies inside one 64-aligned system, so its sums are per system
+        for (int k = 0; k < FAST_E_STRIDE; k++) {
             const double s = block_sum<256>(e[k], red);
-            if (threadIdx.x == 0) a.e_part[blockIdx.x * 7 + k] = s;
+            if (threadIdx.x == 0) a.e_blk[blockIdx.x * FAST_E_STRIDE + k] = s;
         }
     }
-    // per-block integer counts, one atomic each
+    // profiling counters: one atomic per block each
     __shared__ unsigned cc, cn;
     if (threadIdx.x == 0) {
         cc = 0;
         cn = 0;
     }
     __syncthreads();
-    if (ncontact) atomicAdd(&cc, ncontact);
+    if (g == 0 && ncontact) atomicAdd(&cc, ncontact);
     if (nnear) atomicAdd(&cn, nnear);
     __syncthreads();
     if (threadIdx.x == 0) {
-        if (cc) {
-            atomicAdd(&a.contacts[0], (unsigned long long)cc);
-            atomicAdd(&a.contacts[1], (unsigned long long)cc);
-        }
-        if (cn) atomicAdd(a.near_total, (unsigned long long)cn);
+        if (cc) atomicAdd(&a.counters[0], (unsigned long long)cc);
+        if (cn) atomicAdd(&a.counters[1], (unsigned long long)cn);
     }
+}
+
+// Start-of-call drift of the valid balls (each system with its own dt).
+__global__ void k_drift_sys(State s, const int* blk_sys, const int* sys_off, const int* sys_n,
+                            const Phys* sysP, int n_used)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n_used) return;
+    const int sy = blk_sys[b / TB];
+    if (b - sys_off[sy] >= sys_n[sy]) return;
+    const double dt = sysP[sy].dt;
+    const double px = s.f[PX][b], py = s.f[PY][b], pz = s.f[PZ][b];
+    s.f[QX][b] = px;
+    s.f[QY][b] = py;
+    s.f[QZ][b] = pz;
+    const double hx = s.f[VX][b] + .5 * s.f[AX][b] * dt;
+    const double hy = s.f[VY][b] + .5 * s.f[AY][b] * dt;
+    const double hz = s.f[VZ][b] + .5 * s.f[AZ][b] * dt;
+    s.f[HX][b] = hx;
+    s.f[HY][b] = hy;
+    s.f[HZ][b] = hz;
+    s.f[GX][b] = s.f[WX][b] + .5 * s.f[BX][b] * dt;
+    s.f[GY][b] = s.f[WY][b] + .5 * s.f[BY][b] * dt;
+    s.f[GZ][b] = s.f[WZ][b] + .5 * s.f[BZ][b] * dt;
+    s.f[PX][b] = px + hx * dt;
+    s.f[PY][b] = py + hy * dt;
+    s.f[PZ][b] = pz + hz * dt;
 }
 
 }  // namespace fastk

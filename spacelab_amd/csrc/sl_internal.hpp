@@ -1,9 +1,12 @@
 // Internal interfaces between the C ABI (sl_api.hip) and the kernel translation units.
 #pragma once
 
+#include "spacelab/sl_api.h"
 #include "sl_common.hpp"
 
+#include <map>
 #include <string>
+#include <vector>
 
 namespace sl {
 
@@ -47,42 +50,93 @@ struct NearEntry {
     double pe;
 };
 
+// Layout of one or more independent systems in one set of state arrays: system s
+// owns balls [off[s], off[s] + n[s]); off[s] is a multiple of FAST_TILE and the
+// slots up to the next system are padding.  A single simulation is one system.
+struct FastLayout {
+    int n_sys = 0;
+    std::vector<int> n, off, nb;   // balls, first ball, 64-ball blocks per system
+    int n_used = 0;                // ball slots in use (padded)
+    int nb_max = 0;                // largest block count of one system
+};
+
+// Lay out systems of the given sizes (FAST_TILE-aligned offsets).
+FastLayout fast_layout(int n_sys, const int* n_per_sys);
+
+// Optional hooks called around each kernel launch (profiling with HIP events).
+struct LaunchHook {
+    void (*before)(void* user, int kind) = nullptr;   // kind 0: pair/tile kernel, 1: per-ball kernel
+    void (*after)(void* user, int kind) = nullptr;
+    void* user = nullptr;
+};
+
 struct FastBuffers {
-    int n_max = 0, n_pad = 0;
-    int n = 0;                    // system size the tile table was built for
+    int n_pad = 0;                // capacity of the state arrays (balls)
+    int max_sys = 0;
+    FastLayout lay;
+    std::vector<Phys> P;          // per system
+    std::vector<int> blk_sys_h;   // per 64-ball block: system (host copy)
+    std::vector<int> tile_sys_h;  // per tile: system (host copy)
     double* alt_base = nullptr;   // second state set (ping-pong)
     State alt{};
     bool cur_is_alt = false;      // which set holds the current state
     bool drifted = false;         // the next step's drift was already applied
     FastAux aux{};
     double L0 = 1, M0 = 1;        // length and mass units of the mixed path
-    int2* tiles = nullptr;        // (I, J) block pairs, I >= J
-    int n_tiles = 0;
+    int* blk_sys = nullptr;       // device copies of the layout
+    int* sys_off = nullptr;
+    int* sys_n = nullptr;
+    int* sys_nb = nullptr;
+    Phys* sysP = nullptr;
+    int4* tiles = nullptr;        // {I, J, system, first block of system}, I >= J
+    int n_tiles = 0, cap_tiles = 0;
     double* part = nullptr;       // far-field force partials [K][xyz][n_pad]
+    int part_slots = 0;           // K capacity of part
     int* nb_count = nullptr;      // near entries filed this step, per ball
     NearEntry* nb_ent = nullptr;  // [n_pad][FAST_MAX_NEAR]
-    int* flags = nullptr;         // [0] near-list overflow
-    unsigned long long* near_total = nullptr;
+    int* flags = nullptr;         // [0] near-list overflow, [1] non-finite state
+    unsigned long long* counters = nullptr;   // [0] contacts, [1] near entries (profiling)
+    double* e_blk = nullptr;      // per 32-ball block: PE(near), KE, p, L, contacts, nonfinite
+    double* pe_tile = nullptr;    // per tile: far-field PE (write steps)
+    double* h_e_blk = nullptr;    // pinned host copies
+    double* h_pe_tile = nullptr;
+    int precision = 0;
     int split = 0;                // wavefronts per 64x64 tile (0: not tuned yet)
     int split_key = -1;           // (precision, gravity) the choice was tuned for
-    int tile_blocks = 0;          // blocks of the last tile launch (its PE partials)
+    int forced_split = 0;         // 1, 2 or 4 forces the choice
+    bool use_graphs = true;
+    LaunchHook hook{};
+    struct GraphKey {
+        int len, write_last, parity;
+        bool operator<(const GraphKey& o) const
+        {
+            return len != o.len ? len < o.len : write_last != o.write_last ? write_last < o.write_last : parity < o.parity;
+        }
+    };
+    std::map<GraphKey, hipGraphExec_t> graphs;
 };
 
-int fast_alloc(FastBuffers& fb, int n_max, int n_pad);
+constexpr int FAST_E_STRIDE = 10;   // doubles per energy block
+
+int fast_alloc(FastBuffers& fb, int n_pad, int max_sys);
 void fast_free(FastBuffers& fb);
-int fast_max_blocks(int n_max);
-int fast_sync_from(FastBuffers& fb, State& canon, const Phys& P, int n, hipStream_t st, std::string* err);
-int fast_sync_to(FastBuffers& fb, State& canon, int n, hipStream_t st, std::string* err);
-int fast_begin_init(FastBuffers& fb, State& canon, int n, hipStream_t st, std::string* err);
-int fast_drift(FastBuffers& fb, State& canon, int n, double dt, hipStream_t st, std::string* err);
-int fast_pairs(FastBuffers& fb, State& canon, const Phys& P, int n, int precision, int write,
-               double* pe_part, unsigned long long* contacts, hipStream_t st, std::string* err);
-int fast_ball(FastBuffers& fb, State& canon, const Phys& P, int n, int precision, int write, int kick,
-              int drift_next, double* pe_part, double* e_part, unsigned long long* contacts,
-              int* nonfinite, hipStream_t st, int* n_pe_blocks, std::string* err);
+// Build the tile tables and per-ball constants for a layout already placed in canon.
+int fast_configure(FastBuffers& fb, State& canon, const FastLayout& lay, const Phys* P, int precision,
+                   hipStream_t st, std::string* err);
+// New physics parameters for the same layout (re-derives per-ball constants).
+int fast_set_phys(FastBuffers& fb, State& canon, const Phys* P, int precision, hipStream_t st, std::string* err);
+// Bring the canonical state set up to date (the fast path may hold it in the other set).
+int fast_sync_to(FastBuffers& fb, State& canon, hipStream_t st, std::string* err);
+// Step-0 pass: forces of the current positions and the energies (one per system).
+int fast_init(FastBuffers& fb, State& canon, sl_energy* e_sys, hipStream_t st, std::string* err);
+// n_steps steps; the energies of write steps go to e_writes[w * n_sys + s] for the first
+// cap writes.  Returns the number of write steps or a negative error.
+int fast_steps(FastBuffers& fb, State& canon, int n_steps, long long* step_count, int write_every,
+               sl_energy* e_writes, int cap, hipStream_t st, std::string* err);
 int fast_check(FastBuffers& fb, hipStream_t st, std::string* err);
-int fast_energy_blocks(int n);   // energy partial blocks written by the fast per-ball kernel
+int fast_max_tile_blocks(int n_pad);
+void fast_clear_graphs(FastBuffers& fb);
 hipError_t fast_reset_counters(FastBuffers& fb, hipStream_t st);
-long long fast_read_near_total(FastBuffers& fb, hipStream_t st);
+int fast_read_counters(FastBuffers& fb, hipStream_t st, long long* contacts, long long* near_pairs);
 
 }  // namespace sl

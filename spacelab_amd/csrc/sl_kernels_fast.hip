@@ -1,10 +1,12 @@
-// Fast path: every unique pair evaluated once (Newton's third law), far-field
-// algebra without square roots, near pairs in the reference's arithmetic.
-// Host-side management and launches; the kernels are in sl_fast_kernels.hpp.
+// Fast path engine: every unique pair evaluated once (Newton's third law), far-field
+// algebra without square roots, near pairs in the reference's arithmetic.  Runs one
+// system or a batch of independent systems laid out in one set of state arrays; the
+// kernels are in sl_fast_kernels.hpp.
 //
-// Step structure (two kernels per step once running): k_tiles (pair triangle
-// in 64x64 tiles, one wavefront each) then k_ball (per-ball sums, near pairs,
-// kick, energies and the next step's drift), ping-ponging between two state sets.
+// Per step: k_tiles (the pair triangles of every system in 64x64 tiles) then k_ball
+// (per-ball sums, near pairs, kick, energies and the next step's drift), ping-ponging
+// between two state sets.  Runs of steps between write steps are captured once as
+// HIP graphs and replayed.
 #include "spacelab/sl_api.h"
 #include "sl_fast_kernels.hpp"
 #include "sl_internal.hpp"
@@ -20,17 +22,26 @@ using namespace fastk;
 
 namespace {
 
-// Padding balls [n, n_pad): far apart, zero cohesion and gravity weights, so the
-// tile kernel can read whole 64-ball blocks without bounds tests.
-__global__ void k_pad(State s, FastAux aux, int n, int n_pad)
+// Padding slots (outside every system's balls): far apart, zero radius and zero
+// cohesion and gravity weights, so the tile kernel can read whole 64-ball blocks
+// without tests.  With R = 0 a padding pair's force and energy terms are exactly
+// zero (s = t, so log(s/t) = 0), which keeps a system's sums independent of where
+// its padding lies.
+__global__ void k_pad_sys(State s, FastAux aux, const int* blk_sys, const int* sys_off, const int* sys_n,
+                          int n_used, int n_pad)
 {
-    const int b = n + blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n_pad) return;
-    const double x = 1e12 * (double)(b + 1);
-    s.f[PX][b] = x;
+    if (b < n_used) {
+        const int sy = blk_sys[b / TB];
+        if (b - sys_off[sy] < sys_n[sy]) return;
+    }
+    s.f[PX][b] = 1e12 * (double)(b + 1);
     s.f[PY][b] = 0;
     s.f[PZ][b] = 0;
-    s.f[RAD][b] = 1e-30;
+    s.f[RAD][b] = 0;
+    s.f[MASS][b] = 1.0;
+    s.f[MOI][b] = 1.0;
     aux.q[b] = 0;
     aux.g[b] = 0;
     aux.r32[b] = 0.f;
@@ -67,143 +78,16 @@ __global__ void k_probe_rot(int* out)
     out[128 + l] = (int)rotk((double)l, 31);
 }
 
+constexpr int GRAPH_MAX_STEPS = 128;   // steps per captured graph (two kernels each)
+
 State& cur_state(FastBuffers& fb, State& canon) { return fb.cur_is_alt ? fb.alt : canon; }
 State& nxt_state(FastBuffers& fb, State& canon) { return fb.cur_is_alt ? canon : fb.alt; }
 
-int nblocks64(int n) { return (n + TB - 1) / TB; }
-
-}  // namespace
-
-int fast_max_blocks(int n_max)
+int err_hip(std::string* err, const char* what)
 {
-    const int nb = nblocks64(n_max);
-    const long long tiles = (long long)nb * (nb + 1) / 2;
-    // the finest tile split puts one tile per block
-    return (int)tiles + (n_max + BPB - 1) / BPB;
+    if (err) *err = std::string("fast path: ") + what;
+    return SL_E_HIP;
 }
-
-int fast_energy_blocks(int n) { return (n + BPB - 1) / BPB; }
-
-int fast_alloc(FastBuffers& fb, int n_max, int n_pad)
-{
-    fb.n_max = n_max;
-    fb.n_pad = n_pad;
-    const int nb = nblocks64(n_max);
-    const long long tiles = (long long)nb * (nb + 1) / 2;
-    if (hipMalloc(&fb.alt_base, sizeof(double) * (size_t)NFIELD * n_pad) != hipSuccess) return -1;
-    if (hipMalloc(&fb.aux.q, sizeof(double) * n_pad) != hipSuccess) return -1;
-    if (hipMalloc(&fb.aux.g, sizeof(double) * n_pad) != hipSuccess) return -1;
-    if (hipMalloc(&fb.aux.r32, sizeof(float) * n_pad) != hipSuccess) return -1;
-    if (hipMalloc(&fb.aux.q32, sizeof(float) * n_pad) != hipSuccess) return -1;
-    if (hipMalloc(&fb.aux.g32, sizeof(float) * n_pad) != hipSuccess) return -1;
-    if (hipMalloc(&fb.tiles, sizeof(int2) * (size_t)tiles) != hipSuccess) return -1;
-    if (hipMalloc(&fb.part, sizeof(double) * (size_t)nb * 3 * n_pad) != hipSuccess) return -1;
-    if (hipMalloc(&fb.nb_count, sizeof(int) * n_pad) != hipSuccess) return -1;
-    if (hipMalloc(&fb.nb_ent, sizeof(NearEntry) * (size_t)n_pad * MAXNB) != hipSuccess) return -1;
-    if (hipMalloc(&fb.flags, sizeof(int) * 4) != hipSuccess) return -1;
-    if (hipMalloc(&fb.near_total, sizeof(unsigned long long)) != hipSuccess) return -1;
-    if (hipMemset(fb.alt_base, 0, sizeof(double) * (size_t)NFIELD * n_pad) != hipSuccess) return -1;
-    if (hipMemset(fb.nb_count, 0, sizeof(int) * n_pad) != hipSuccess) return -1;
-    if (hipMemset(fb.flags, 0, sizeof(int) * 4) != hipSuccess) return -1;
-    if (hipMemset(fb.near_total, 0, sizeof(unsigned long long)) != hipSuccess) return -1;
-    for (int f = 0; f < NFIELD; f++) fb.alt.f[f] = fb.alt_base + (size_t)f * n_pad;
-    return 0;
-}
-
-void fast_free(FastBuffers& fb)
-{
-    void* ptrs[] = {fb.alt_base, fb.aux.q, fb.aux.g, fb.aux.r32, fb.aux.q32, fb.aux.g32, fb.tiles,
-                    fb.part, fb.nb_count, fb.nb_ent, fb.flags, fb.near_total};
-    for (void* p : ptrs)
-        if (p) (void)hipFree(p);
-    fb = FastBuffers{};
-}
-
-int fast_sync_from(FastBuffers& fb, State& canon, const Phys& P, int n, hipStream_t st, std::string* err)
-{
-    auto fail = [&](const char* what) {
-        if (err) *err = std::string("fast path set-up: ") + what;
-        return SL_E_HIP;
-    };
-    if (!fb.alt_base) return 0;
-    // the radius/mass/moi arrays are shared by both state sets
-    fb.alt.f[RAD] = canon.f[RAD];
-    fb.alt.f[MASS] = canon.f[MASS];
-    fb.alt.f[MOI] = canon.f[MOI];
-    fb.cur_is_alt = false;
-    fb.drifted = false;
-    // units of the mixed path: median radius and mass
-    std::vector<double> R(n), m(n);
-    if (hipMemcpyAsync(R.data(), canon.f[RAD], sizeof(double) * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(m.data(), canon.f[MASS], sizeof(double) * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-        return fail("read radii");
-    std::nth_element(R.begin(), R.begin() + n / 2, R.end());
-    std::nth_element(m.begin(), m.begin() + n / 2, m.end());
-    fb.L0 = R[n / 2];
-    fb.M0 = m[n / 2];
-    const double C = P.Ha / 6 * 64;
-    k_aux<<<(n + 255) / 256, 256, 0, st>>>(canon, fb.aux, n, std::sqrt(C), std::sqrt(P.G), 1.0 / fb.L0, 1.0 / fb.M0);
-    const int np = fb.n_pad;
-    if (np > n) {
-        k_pad<<<(np - n + 255) / 256, 256, 0, st>>>(canon, fb.aux, n, np);
-        k_pad<<<(np - n + 255) / 256, 256, 0, st>>>(fb.alt, fb.aux, n, np);
-    }
-    if (hipGetLastError() != hipSuccess) return fail("aux kernels");
-    if (fb.n != n) {
-        const int nb = nblocks64(n);
-        std::vector<int2> t;
-        t.reserve((size_t)nb * (nb + 1) / 2);
-        for (int I = 0; I < nb; I++)
-            for (int J = 0; J <= I; J++) t.push_back(make_int2(I, J));
-        if (hipMemcpyAsync(fb.tiles, t.data(), sizeof(int2) * t.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess)
-            return fail("tile table");
-        fb.n_tiles = (int)t.size();
-        fb.n = n;
-        fb.split = 0;   // re-tune for the new size
-    }
-    if (launch_zero_i32(fb.nb_count, fb.n_pad, st) != hipSuccess || launch_zero_i32(fb.flags, 4, st) != hipSuccess)
-        return fail("clear near lists");
-    return 0;
-}
-
-int fast_sync_to(FastBuffers& fb, State& canon, int n, hipStream_t st, std::string* err)
-{
-    if (!fb.cur_is_alt) return 0;
-    k_copy_fields<<<(n + 255) / 256, 256, 0, st>>>(canon, fb.alt, n, PX, RAD);
-    if (hipGetLastError() != hipSuccess) {
-        if (err) *err = "fast path: copy state";
-        return SL_E_HIP;
-    }
-    fb.cur_is_alt = false;
-    return 0;
-}
-
-int fast_begin_init(FastBuffers& fb, State& canon, int n, hipStream_t st, std::string* err)
-{
-    if (launch_copy_prev(cur_state(fb, canon), n, st) != hipSuccess) {
-        if (err) *err = "fast path: copy positions";
-        return SL_E_HIP;
-    }
-    fb.drifted = false;
-    return 0;
-}
-
-int fast_drift(FastBuffers& fb, State& canon, int n, double dt, hipStream_t st, std::string* err)
-{
-    if (fb.drifted) {
-        fb.drifted = false;
-        return 0;
-    }
-    if (launch_drift(cur_state(fb, canon), n, dt, st) != hipSuccess) {
-        if (err) *err = "fast path: drift";
-        return SL_E_HIP;
-    }
-    return 0;
-}
-
-namespace {
 
 template <bool WRITE, bool GRAV, int SPLIT>
 void launch_tiles(const TileArgs& a, int precision, hipStream_t st)
@@ -235,13 +119,7 @@ void launch_tiles_any(const TileArgs& a, int split, int precision, bool write, b
         launch_tiles_split<1>(a, precision, write, grav, st);
 }
 
-int tile_blocks_for(int n_tiles, int split)
-{
-    const int tpb = NW / split;
-    return (n_tiles + tpb - 1) / tpb;
-}
-
-TileArgs tile_args(FastBuffers& fb, State& c, const Phys& P, double* pe_part)
+TileArgs tile_args(FastBuffers& fb, State& c)
 {
     TileArgs a;
     a.px = c.f[PX];
@@ -250,10 +128,11 @@ TileArgs tile_args(FastBuffers& fb, State& c, const Phys& P, double* pe_part)
     a.R = c.f[RAD];
     a.aux = fb.aux;
     a.cur = c;
-    a.P = P;
+    a.sysP = fb.sysP;
     a.n_pad = fb.n_pad;
     a.n_tiles = fb.n_tiles;
     a.tiles = fb.tiles;
+    const Phys& P = fb.P[0];   // G, Ha and h_min are common to a batch
     a.L0inv2 = 1.0 / (fb.L0 * fb.L0);
     a.Kv = (P.Ha / 6 * 64) / (fb.L0 * fb.L0);
     a.Kg = P.G * fb.M0 * fb.M0;
@@ -261,125 +140,451 @@ TileArgs tile_args(FastBuffers& fb, State& c, const Phys& P, double* pe_part)
     a.nb_count = fb.nb_count;
     a.nb_ent = fb.nb_ent;
     a.flags = fb.flags;
-    a.pe_part = pe_part;
+    a.pe_tile = fb.pe_tile;
     return a;
 }
 
-// Autotuner: the number of wavefronts sharing one 64x64 tile trades per-wave
-// set-up against occupancy and load balance, which depend on the system size and
-// the machine.  Time each choice on the live state (HIP events on the context's
-// stream, best of three launches) and keep the fastest; the near lists these
-// trial launches fill are cleared again.  SL_TILE_SPLIT=1|2|4 forces a choice.
-int tune_split(FastBuffers& fb, const TileArgs& a, int precision, bool grav, hipStream_t st)
+bool gravity_on(const FastBuffers& fb) { return fb.P[0].far_gravity != 0; }
+
+int split_key(const FastBuffers& fb) { return fb.precision * 2 + (gravity_on(fb) ? 1 : 0); }
+
+// The tile split has been chosen for the current precision and gravity setting.
+bool split_ready(const FastBuffers& fb) { return fb.split != 0 && fb.split_key == split_key(fb); }
+
+// Autotuner: the number of wavefronts sharing one 64x64 tile trades per-wave set-up
+// against occupancy and load balance, which depend on the systems' sizes and the
+// machine.  Each choice is warmed up once and then timed over five launches on the
+// live state (HIP events on the engine's stream); the fastest is kept.  The near
+// entries these trial launches file are cleared again.
+int tune_split(FastBuffers& fb, const TileArgs& a, hipStream_t st)
 {
-    const char* env = std::getenv("SL_TILE_SPLIT");
-    if (env) {
-        const int v = std::atoi(env);
-        if (v == 1 || v == 2 || v == 4) return v;
-    }
+    if (fb.forced_split == 1 || fb.forced_split == 2 || fb.forced_split == 4) return fb.forced_split;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return 1;
     if (hipEventCreate(&e1) != hipSuccess) {
-        hipEventDestroy(e0);
+        (void)hipEventDestroy(e0);
         return 1;
     }
+    const bool grav = gravity_on(fb);
     int best = 1;
     float best_ms = 1e30f;
     const int choices[3] = {1, 2, 4};
     for (int c : choices) {
-        float ms_min = 1e30f;
-        for (int rep = 0; rep < 3; rep++) {
-            hipEventRecord(e0, st);
-            launch_tiles_any(a, c, precision, false, grav, st);
-            hipEventRecord(e1, st);
-            if (hipEventSynchronize(e1) != hipSuccess) break;
-            float ms = 0;
-            hipEventElapsedTime(&ms, e0, e1);
-            ms_min = ms < ms_min ? ms : ms_min;
-            launch_zero_i32(fb.nb_count, fb.n_pad, st);
+        launch_tiles_any(a, c, fb.precision, false, grav, st);
+        (void)launch_zero_i32(fb.nb_count, fb.n_pad, st);
+        (void)hipEventRecord(e0, st);
+        for (int rep = 0; rep < 5; rep++) {
+            launch_tiles_any(a, c, fb.precision, false, grav, st);
+            (void)launch_zero_i32(fb.nb_count, fb.n_pad, st);
         }
-        if (ms_min < best_ms) {
-            best_ms = ms_min;
+        (void)hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess) break;
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best_ms) {
+            best_ms = ms;
             best = c;
         }
     }
-    launch_zero_i32(fb.nb_count, fb.n_pad, st);
-    launch_zero_i32(fb.flags, 4, st);
-    hipStreamSynchronize(st);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)launch_zero_i32(fb.flags, 4, st);
+    (void)hipStreamSynchronize(st);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return best;
 }
 
-}  // namespace
-
-int fast_pairs(FastBuffers& fb, State& canon, const Phys& P, int n, int precision, int write,
-               double* pe_part, unsigned long long* contacts, hipStream_t st, std::string* err)
+int launch_tiles_step(FastBuffers& fb, State& canon, bool write, hipStream_t st, std::string* err)
 {
-    (void)contacts;
-    (void)n;
     State& c = cur_state(fb, canon);
-    const TileArgs a = tile_args(fb, c, P, pe_part);
-    const bool g = P.far_gravity != 0;
-    const int key = precision * 2 + (g ? 1 : 0);
-    if (fb.split == 0 || fb.split_key != key) {
-        fb.split = tune_split(fb, a, precision, g, st);
-        fb.split_key = key;
+    const TileArgs a = tile_args(fb, c);
+    if (!split_ready(fb)) {
+        fb.split = tune_split(fb, a, st);
+        fb.split_key = split_key(fb);
+        fast_clear_graphs(fb);
     }
-    launch_tiles_any(a, fb.split, precision, write != 0, g, st);
-    fb.tile_blocks = tile_blocks_for(fb.n_tiles, fb.split);
-    if (hipGetLastError() != hipSuccess) {
-        if (err) *err = "fast path: tile kernel launch";
-        return SL_E_HIP;
-    }
+    if (fb.hook.before) fb.hook.before(fb.hook.user, 0);
+    launch_tiles_any(a, fb.split, fb.precision, write, gravity_on(fb), st);
+    if (fb.hook.after) fb.hook.after(fb.hook.user, 0);
+    if (hipGetLastError() != hipSuccess) return err_hip(err, "tile kernel launch");
     return 0;
 }
 
-int fast_ball(FastBuffers& fb, State& canon, const Phys& P, int n, int precision, int write, int kick,
-              int drift_next, double* pe_part, double* e_part, unsigned long long* contacts,
-              int* nonfinite, hipStream_t st, int* n_pe_blocks, std::string* err)
+int launch_ball_step(FastBuffers& fb, State& canon, bool write, bool kick, bool drift_next, hipStream_t st,
+                     std::string* err)
 {
-    const int tile_blocks = fb.tile_blocks;
     BallArgs a;
     a.cur = cur_state(fb, canon);
     a.nxt = nxt_state(fb, canon);
-    a.P = P;
-    a.n = n;
+    a.sysP = fb.sysP;
+    a.blk_sys = fb.blk_sys;
+    a.sys_off = fb.sys_off;
+    a.sys_n = fb.sys_n;
+    a.sys_nb = fb.sys_nb;
+    a.n_used = fb.lay.n_used;
     a.n_pad = fb.n_pad;
-    a.nblk = nblocks64(n);
     a.part = fb.part;
     a.nb_count = fb.nb_count;
     a.nb_ent = fb.nb_ent;
     a.write = write;
     a.kick = kick;
     a.drift_next = drift_next;
-    a.compensated = precision == SL_PREC_MIXED;
-    a.pe_part = pe_part + tile_blocks;
-    a.e_part = e_part;
-    a.contacts = contacts;
-    a.near_total = fb.near_total;
-    a.nonfinite = nonfinite;
-    const int blocks = (n + BPB - 1) / BPB;
+    a.compensated = fb.precision == SL_PREC_MIXED;
+    a.e_blk = fb.e_blk;
+    a.flags = fb.flags;
+    a.counters = fb.counters;
+    const int blocks = (fb.lay.n_used + BPB - 1) / BPB;
+    if (fb.hook.before) fb.hook.before(fb.hook.user, 1);
     k_ball<<<blocks, 256, 0, st>>>(a);
-    if (hipGetLastError() != hipSuccess) {
-        if (err) *err = "fast path: ball kernel launch";
-        return SL_E_HIP;
-    }
+    if (fb.hook.after) fb.hook.after(fb.hook.user, 1);
+    if (hipGetLastError() != hipSuccess) return err_hip(err, "ball kernel launch");
     fb.cur_is_alt = !fb.cur_is_alt;
-    fb.drifted = drift_next != 0;
-    if (n_pe_blocks) *n_pe_blocks = write ? tile_blocks + blocks : 0;
+    fb.drifted = drift_next;
     return 0;
+}
+
+int launch_drift_sys(FastBuffers& fb, State& canon, hipStream_t st, std::string* err)
+{
+    k_drift_sys<<<(fb.lay.n_used + 255) / 256, 256, 0, st>>>(cur_state(fb, canon), fb.blk_sys, fb.sys_off,
+                                                             fb.sys_n, fb.sysP, fb.lay.n_used);
+    if (hipGetLastError() != hipSuccess) return err_hip(err, "drift");
+    return 0;
+}
+
+// Read back the partials of a write step and sum them per system, in a fixed order:
+// tiles in table order, then 32-ball blocks in ascending order.
+int read_energies(FastBuffers& fb, sl_energy* out, hipStream_t st, std::string* err)
+{
+    const int nblk = (fb.lay.n_used + BPB - 1) / BPB;
+    if (hipMemcpyAsync(fb.h_pe_tile, fb.pe_tile, sizeof(double) * fb.n_tiles, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(fb.h_e_blk, fb.e_blk, sizeof(double) * FAST_E_STRIDE * nblk, hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return err_hip(err, "energy read-back");
+    const int ns = fb.lay.n_sys;
+    std::vector<sl_energy> e(ns);
+    for (auto& x : e) x = sl_energy{};
+    for (int t = 0; t < fb.n_tiles; t++) e[fb.tile_sys_h[t]].PE += fb.h_pe_tile[t];
+    for (int b = 0; b < nblk; b++) {
+        const int s = fb.blk_sys_h[(b * BPB) / TB];
+        const double* v = fb.h_e_blk + (size_t)b * FAST_E_STRIDE;
+        sl_energy& x = e[s];
+        x.PE += v[0];
+        x.KE += v[1];
+        x.p[0] += v[2];
+        x.p[1] += v[3];
+        x.p[2] += v[4];
+        x.L[0] += v[5];
+        x.L[1] += v[6];
+        x.L[2] += v[7];
+        x.contacts += (long long)v[8];
+        if (v[9] > 0) x.nonfinite = 1;
+    }
+    if (out)
+        for (int s = 0; s < ns; s++) out[s] = e[s];
+    return 0;
+}
+
+// One step launched directly.
+int direct_step(FastBuffers& fb, State& canon, bool write, bool drift_next, hipStream_t st, std::string* err)
+{
+    int rc;
+    if (!fb.drifted && (rc = launch_drift_sys(fb, canon, st, err))) return rc;
+    fb.drifted = false;
+    if ((rc = launch_tiles_step(fb, canon, write, st, err))) return rc;
+    return launch_ball_step(fb, canon, write, true, drift_next, st, err);
+}
+
+// A run of `len` steps that starts drifted and ends drifted (the last may be a
+// write step), captured once per (length, write, state parity) and replayed.
+int graph_segment(FastBuffers& fb, State& canon, int len, bool write_last, hipStream_t st, std::string* err)
+{
+    const FastBuffers::GraphKey key{len, write_last ? 1 : 0, fb.cur_is_alt ? 1 : 0};
+    auto it = fb.graphs.find(key);
+    if (it == fb.graphs.end()) {
+        if (fb.graphs.size() >= 16) fast_clear_graphs(fb);
+        const bool parity0 = fb.cur_is_alt;
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return err_hip(err, "begin graph capture");
+        int rc = 0;
+        for (int k = 0; k < len && !rc; k++) {
+            const bool w = write_last && k == len - 1;
+            rc = launch_tiles_step(fb, canon, w, st, err);
+            if (!rc) rc = launch_ball_step(fb, canon, w, true, true, st, err);
+        }
+        const hipError_t ec = hipStreamEndCapture(st, &g);
+        fb.cur_is_alt = parity0;   // capture recorded the launches; nothing ran yet
+        fb.drifted = true;
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess || !g) return err_hip(err, "end graph capture");
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess || !ex) return err_hip(err, "instantiate graph");
+        it = fb.graphs.emplace(key, ex).first;
+    }
+    if (hipGraphLaunch(it->second, st) != hipSuccess) return err_hip(err, "graph launch");
+    if (len & 1) fb.cur_is_alt = !fb.cur_is_alt;
+    fb.drifted = true;
+    return 0;
+}
+
+}  // namespace
+
+FastLayout fast_layout(int n_sys, const int* n_per_sys)
+{
+    FastLayout L;
+    L.n_sys = n_sys;
+    int off = 0;
+    for (int s = 0; s < n_sys; s++) {
+        const int n = n_per_sys[s];
+        const int nb = (n + TB - 1) / TB;
+        L.n.push_back(n);
+        L.off.push_back(off);
+        L.nb.push_back(nb);
+        L.nb_max = std::max(L.nb_max, nb);
+        off += nb * TB;
+    }
+    L.n_used = off;
+    return L;
+}
+
+int fast_max_tile_blocks(int n_pad)
+{
+    const long long nb = (n_pad + TB - 1) / TB;
+    return (int)std::min<long long>(nb * (nb + 1) / 2, 1 << 30);
+}
+
+int fast_alloc(FastBuffers& fb, int n_pad, int max_sys)
+{
+    fb.n_pad = n_pad;
+    fb.max_sys = max_sys;
+    const int nblocks = (n_pad + TB - 1) / TB;
+    if (hipMalloc(&fb.alt_base, sizeof(double) * (size_t)NFIELD * n_pad) != hipSuccess) return -1;
+    if (hipMalloc(&fb.aux.q, sizeof(double) * n_pad) != hipSuccess) return -1;
+    if (hipMalloc(&fb.aux.g, sizeof(double) * n_pad) != hipSuccess) return -1;
+    if (hipMalloc(&fb.aux.r32, sizeof(float) * n_pad) != hipSuccess) return -1;
+    if (hipMalloc(&fb.aux.q32, sizeof(float) * n_pad) != hipSuccess) return -1;
+    if (hipMalloc(&fb.aux.g32, sizeof(float) * n_pad) != hipSuccess) return -1;
+    if (hipMalloc(&fb.blk_sys, sizeof(int) * nblocks) != hipSuccess) return -1;
+    if (hipMalloc(&fb.sys_off, sizeof(int) * max_sys) != hipSuccess) return -1;
+    if (hipMalloc(&fb.sys_n, sizeof(int) * max_sys) != hipSuccess) return -1;
+    if (hipMalloc(&fb.sys_nb, sizeof(int) * max_sys) != hipSuccess) return -1;
+    if (hipMalloc(&fb.sysP, sizeof(Phys) * max_sys) != hipSuccess) return -1;
+    if (hipMalloc(&fb.nb_count, sizeof(int) * n_pad) != hipSuccess) return -1;
+    if (hipMalloc(&fb.nb_ent, sizeof(NearEntry) * (size_t)n_pad * MAXNB) != hipSuccess) return -1;
+    if (hipMalloc(&fb.flags, sizeof(int) * 4) != hipSuccess) return -1;
+    if (hipMalloc(&fb.counters, sizeof(unsigned long long) * 2) != hipSuccess) return -1;
+    const int eblocks = (n_pad + BPB - 1) / BPB;
+    if (hipMalloc(&fb.e_blk, sizeof(double) * FAST_E_STRIDE * eblocks) != hipSuccess) return -1;
+    if (hipHostMalloc(&fb.h_e_blk, sizeof(double) * FAST_E_STRIDE * eblocks, 0) != hipSuccess) return -1;
+    if (hipMemset(fb.alt_base, 0, sizeof(double) * (size_t)NFIELD * n_pad) != hipSuccess) return -1;
+    if (hipMemset(fb.nb_count, 0, sizeof(int) * n_pad) != hipSuccess) return -1;
+    if (hipMemset(fb.flags, 0, sizeof(int) * 4) != hipSuccess) return -1;
+    if (hipMemset(fb.counters, 0, sizeof(unsigned long long) * 2) != hipSuccess) return -1;
+    for (int f = 0; f < NFIELD; f++) fb.alt.f[f] = fb.alt_base + (size_t)f * n_pad;
+    return 0;
+}
+
+void fast_clear_graphs(FastBuffers& fb)
+{
+    for (auto& kv : fb.graphs) (void)hipGraphExecDestroy(kv.second);
+    fb.graphs.clear();
+}
+
+void fast_free(FastBuffers& fb)
+{
+    fast_clear_graphs(fb);
+    void* ptrs[] = {fb.alt_base, fb.aux.q, fb.aux.g, fb.aux.r32, fb.aux.q32, fb.aux.g32, fb.blk_sys,
+                    fb.sys_off, fb.sys_n, fb.sys_nb, fb.sysP, fb.tiles, fb.part, fb.nb_count, fb.nb_ent,
+                    fb.flags, fb.counters, fb.e_blk, fb.pe_tile};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (fb.h_e_blk) (void)hipHostFree(fb.h_e_blk);
+    if (fb.h_pe_tile) (void)hipHostFree(fb.h_pe_tile);
+    fb = FastBuffers{};
+}
+
+int fast_set_phys(FastBuffers& fb, State& canon, const Phys* P, int precision, hipStream_t st, std::string* err)
+{
+    const FastLayout& L = fb.lay;
+    fb.P.assign(P, P + L.n_sys);
+    fb.precision = precision;
+    if (hipMemcpyAsync(fb.sysP, fb.P.data(), sizeof(Phys) * L.n_sys, hipMemcpyHostToDevice, st) != hipSuccess)
+        return err_hip(err, "physics table");
+    // units of the mixed path: median radius and mass of the balls in use
+    std::vector<double> R(L.n_used), m(L.n_used);
+    if (hipMemcpyAsync(R.data(), canon.f[RAD], sizeof(double) * L.n_used, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(m.data(), canon.f[MASS], sizeof(double) * L.n_used, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return err_hip(err, "read radii");
+    std::vector<double> Rv, mv;
+    for (int s = 0; s < L.n_sys; s++)
+        for (int b = 0; b < L.n[s]; b++) {
+            Rv.push_back(R[L.off[s] + b]);
+            mv.push_back(m[L.off[s] + b]);
+        }
+    const size_t h = Rv.size() / 2;
+    std::nth_element(Rv.begin(), Rv.begin() + h, Rv.end());
+    std::nth_element(mv.begin(), mv.begin() + h, mv.end());
+    fb.L0 = Rv[h];
+    fb.M0 = mv[h];
+    const double C = P[0].Ha / 6 * 64;
+    k_aux<<<(L.n_used + 255) / 256, 256, 0, st>>>(canon, fb.aux, L.n_used, std::sqrt(C), std::sqrt(P[0].G),
+                                                  1.0 / fb.L0, 1.0 / fb.M0);
+    k_pad_sys<<<(fb.n_pad + 255) / 256, 256, 0, st>>>(canon, fb.aux, fb.blk_sys, fb.sys_off, fb.sys_n, L.n_used,
+                                                      fb.n_pad);
+    k_pad_sys<<<(fb.n_pad + 255) / 256, 256, 0, st>>>(fb.alt, fb.aux, fb.blk_sys, fb.sys_off, fb.sys_n, L.n_used,
+                                                      fb.n_pad);
+    if (hipGetLastError() != hipSuccess) return err_hip(err, "per-ball constants");
+    fast_clear_graphs(fb);
+    return 0;
+}
+
+int fast_configure(FastBuffers& fb, State& canon, const FastLayout& lay, const Phys* P, int precision,
+                   hipStream_t st, std::string* err)
+{
+    if (lay.n_used > fb.n_pad || lay.n_sys > fb.max_sys || lay.n_sys < 1) {
+        if (err) *err = "fast path: layout exceeds the allocated capacity";
+        return SL_E_ARG;
+    }
+    const bool same = fb.lay.n_sys == lay.n_sys && fb.lay.n == lay.n;
+    fb.lay = lay;
+    // the radius/mass/moi arrays are shared by both state sets
+    fb.alt.f[RAD] = canon.f[RAD];
+    fb.alt.f[MASS] = canon.f[MASS];
+    fb.alt.f[MOI] = canon.f[MOI];
+    fb.cur_is_alt = false;
+    fb.drifted = false;
+    if (!same) {
+        const int nblocks = lay.n_used / TB;
+        fb.blk_sys_h.assign(std::max(nblocks, 1), 0);
+        std::vector<int4> tiles;
+        fb.tile_sys_h.clear();
+        for (int s = 0; s < lay.n_sys; s++) {
+            const int b0 = lay.off[s] / TB;
+            for (int k = 0; k < lay.nb[s]; k++) fb.blk_sys_h[b0 + k] = s;
+            for (int I = 0; I < lay.nb[s]; I++)
+                for (int J = 0; J <= I; J++) {
+                    tiles.push_back(make_int4(b0 + I, b0 + J, s, b0));
+                    fb.tile_sys_h.push_back(s);
+                }
+        }
+        const int nt = (int)tiles.size();
+        if (nt > fb.cap_tiles) {
+            if (fb.tiles) (void)hipFree(fb.tiles);
+            if (fb.pe_tile) (void)hipFree(fb.pe_tile);
+            if (fb.h_pe_tile) (void)hipHostFree(fb.h_pe_tile);
+            fb.tiles = nullptr;
+            fb.pe_tile = nullptr;
+            fb.h_pe_tile = nullptr;
+            fb.cap_tiles = 0;
+            if (hipMalloc(&fb.tiles, sizeof(int4) * nt) != hipSuccess ||
+                hipMalloc(&fb.pe_tile, sizeof(double) * nt) != hipSuccess ||
+                hipHostMalloc(&fb.h_pe_tile, sizeof(double) * nt, 0) != hipSuccess)
+                return err_hip(err, "tile table allocation");
+            fb.cap_tiles = nt;
+        }
+        if (lay.nb_max > fb.part_slots) {
+            if (fb.part) (void)hipFree(fb.part);
+            fb.part = nullptr;
+            fb.part_slots = 0;
+            if (hipMalloc(&fb.part, sizeof(double) * (size_t)lay.nb_max * 3 * fb.n_pad) != hipSuccess)
+                return err_hip(err, "partial buffer allocation");
+            fb.part_slots = lay.nb_max;
+        }
+        if (hipMemcpyAsync(fb.tiles, tiles.data(), sizeof(int4) * nt, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(fb.blk_sys, fb.blk_sys_h.data(), sizeof(int) * fb.blk_sys_h.size(), hipMemcpyHostToDevice,
+                           st) != hipSuccess ||
+            hipMemcpyAsync(fb.sys_off, lay.off.data(), sizeof(int) * lay.n_sys, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(fb.sys_n, lay.n.data(), sizeof(int) * lay.n_sys, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(fb.sys_nb, lay.nb.data(), sizeof(int) * lay.n_sys, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return err_hip(err, "layout tables");
+        fb.n_tiles = nt;
+        fb.split = 0;   // re-tune for the new sizes
+    }
+    if (launch_zero_i32(fb.nb_count, fb.n_pad, st) != hipSuccess || launch_zero_i32(fb.flags, 4, st) != hipSuccess)
+        return err_hip(err, "clear near lists");
+    return fast_set_phys(fb, canon, P, precision, st, err);
+}
+
+int fast_sync_to(FastBuffers& fb, State& canon, hipStream_t st, std::string* err)
+{
+    if (!fb.cur_is_alt) return 0;
+    k_copy_fields<<<(fb.lay.n_used + 255) / 256, 256, 0, st>>>(canon, fb.alt, fb.lay.n_used, PX, RAD);
+    if (hipGetLastError() != hipSuccess) return err_hip(err, "copy state");
+    fb.cur_is_alt = false;
+    return 0;
+}
+
+int fast_init(FastBuffers& fb, State& canon, sl_energy* e_sys, hipStream_t st, std::string* err)
+{
+    if (launch_copy_prev(cur_state(fb, canon), fb.lay.n_used, st) != hipSuccess) return err_hip(err, "copy positions");
+    fb.drifted = false;
+    int rc = launch_tiles_step(fb, canon, true, st, err);
+    if (!rc) rc = launch_ball_step(fb, canon, true, false, false, st, err);
+    if (!rc) rc = read_energies(fb, e_sys, st, err);
+    if (!rc) rc = fast_check(fb, st, err);
+    return rc;
+}
+
+int fast_steps(FastBuffers& fb, State& canon, int n_steps, long long* step_count, int write_every,
+               sl_energy* e_writes, int cap, hipStream_t st, std::string* err)
+{
+    const int ns = fb.lay.n_sys;
+    const bool graphs = fb.use_graphs && !fb.hook.before;
+    int nw = 0;
+    int k = 0;
+    auto record = [&](void) -> int {
+        sl_energy* dst = (e_writes && nw < cap) ? e_writes + (size_t)nw * ns : nullptr;
+        const int rc = read_energies(fb, dst, st, err);
+        nw++;
+        return rc;
+    };
+    while (k < n_steps) {
+        const long long s = *step_count + 1;
+        const bool write = write_every > 0 && (s % write_every) == 0;
+        // steps until (and including) the next write step, within this call
+        int until = n_steps - k;
+        if (write_every > 0) {
+            const long long next = ((s + write_every - 1) / write_every) * write_every;
+            until = (int)std::min<long long>(until, next - s + 1);
+        }
+        // graph segments need a drifted start and must not include the call's last step;
+        // long runs are cut into segments of at most GRAPH_MAX_STEPS, replayed repeatedly
+        const int seg = std::min(std::min(until, n_steps - 1 - k), GRAPH_MAX_STEPS);
+        if (graphs && fb.drifted && split_ready(fb) && seg >= 2) {
+            const bool write_last = write_every > 0 && ((s + seg - 1) % write_every) == 0;
+            int rc = graph_segment(fb, canon, seg, write_last, st, err);
+            if (rc) return rc;
+            *step_count += seg;
+            k += seg;
+            if (write_last && (rc = record())) return rc;
+            continue;
+        }
+        int rc = direct_step(fb, canon, write, k + 1 < n_steps, st, err);
+        if (rc) return rc;
+        *step_count = s;
+        k++;
+        if (write && (rc = record())) return rc;
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) return err_hip(err, "synchronize");
+    const int rc = fast_check(fb, st, err);
+    if (rc) return rc;
+    return nw;
 }
 
 int fast_check(FastBuffers& fb, hipStream_t st, std::string* err)
 {
-    int h = 0;
-    if (hipMemcpyAsync(&h, fb.flags, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
-        if (err) *err = "fast path: read flags";
-        return SL_E_HIP;
-    }
-    if (h) {
+    int h[2] = {0, 0};
+    if (hipMemcpyAsync(h, fb.flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return err_hip(err, "read flags");
+    if (h[0]) {
         if (err)
             *err = "near-pair list overflow: a sphere has more than 32 partners within h_min of "
                    "contact; results of this call are invalid";
@@ -388,15 +593,17 @@ int fast_check(FastBuffers& fb, hipStream_t st, std::string* err)
     return 0;
 }
 
-hipError_t fast_reset_counters(FastBuffers& fb, hipStream_t st) { return launch_zero_u64(fb.near_total, 1, st); }
+hipError_t fast_reset_counters(FastBuffers& fb, hipStream_t st) { return launch_zero_u64(fb.counters, 2, st); }
 
-long long fast_read_near_total(FastBuffers& fb, hipStream_t st)
+int fast_read_counters(FastBuffers& fb, hipStream_t st, long long* contacts, long long* near_pairs)
 {
-    unsigned long long h = 0;
-    if (hipMemcpyAsync(&h, fb.near_total, sizeof(h), hipMemcpyDeviceToHost, st) != hipSuccess ||
+    unsigned long long h[2] = {0, 0};
+    if (hipMemcpyAsync(h, fb.counters, sizeof(h), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
-        return -1;
-    return (long long)(h / 2);   // each near pair is listed by both of its balls
+        return SL_E_HIP;
+    if (contacts) *contacts = (long long)h[0];
+    if (near_pairs) *near_pairs = (long long)(h[1] / 2);   // each near pair is filed with both balls
+    return 0;
 }
 
 }  // namespace sl
@@ -413,6 +620,6 @@ extern "C" int sl_probe_rotation(int* out192)
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
         hipMemcpy(out192, d, sizeof(int) * 192, hipMemcpyDeviceToHost) != hipSuccess)
         rc = SL_E_HIP;
-    hipFree(d);
+    (void)hipFree(d);
     return rc;
 }
