@@ -38,7 +38,7 @@ extern "C" {
 #define SL_E_ARG -1       /* invalid argument (NULL, n out of range, bad parameter) */
 #define SL_E_HIP -2       /* HIP runtime error */
 #define SL_E_STATE -3     /* call out of order (e.g. step before upload) */
-#define SL_E_CAPACITY -4  /* a fixed-capacity table overflowed (near-pair list) */
+#define SL_E_CAPACITY -4  /* a capacity was exceeded (sl_gen_bpca found no free placement) */
 #define SL_E_NOGPU -5     /* no usable gfx950 device */
 
 /* precision */

@@ -444,6 +444,7 @@ int sl_init_forces(sl_ctx* c, sl_energy* e)
         const int rc = fast_init(c->fb, c->st, e, c->stream, &c->err);
         if (rc) return rc;
     } else {
+        fast_canon_changed(c->fb);
         HIPCHK(c, launch_copy_prev(c->st, c->n, c->stream));
         HIPCHK(c, launch_zero_i32(c->nonfinite, 1, c->stream));
         int npe = 0;
@@ -472,6 +473,7 @@ int sl_step(sl_ctx* c, int n_steps, int write_every, sl_energy* e_out, int e_cap
         if (rc) return rc;
         return nw;
     }
+    if (n_steps > 0) fast_canon_changed(c->fb);
     int nw = 0;
     for (int k = 0; k < n_steps; k++) {
         c->step++;

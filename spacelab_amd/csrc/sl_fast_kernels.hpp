@@ -1,33 +1,40 @@
 // Device kernels of the fast path (included by sl_kernels_fast.hip).
 //
-//   k_tiles_*  SPLIT wavefronts per 64x64 tile (I, J) of the pair triangle, I >= J,
-//              each sweeping 64/SPLIT of the tile's diagonals.  Lane l holds ball
-//              i = 64 I + l in registers.  The 64 balls of block J are staged in
-//              LDS twice (slots 0..63 and 64..127), so at sweep step s lane l reads
-//              slot l + s -- ball j = (l + s) mod 64 -- at base + s*stride, with no
-//              index arithmetic.  The reaction force on j is accumulated in a
-//              register that travels with j: after every step the accumulators
-//              rotate one lane down (DPP wave_rol:1), so lane l always holds the
-//              sum of the j it meets next; no LDS atomics or cross-lane reductions.
+//   k_tiles_*  SPLIT wavefronts per 64x64 tile (I, J) of a system's pair triangle,
+//              I >= J, each sweeping 64/SPLIT of the tile's diagonals.  Lane l
+//              holds ball i = 64 I + l in registers.  The 64 balls of block J are
+//              staged in LDS twice (slots 0..63 and 64..127), so at sweep step s lane
+//              l reads slot l + s -- ball j = (l + s) mod 64 -- at base + s*stride,
+//              with no index arithmetic.  The reaction force on j is accumulated in
+//              a register that travels with j: after every step the accumulators
+//              rotate one lane down (DPP wave_rol:1), so lane l always holds the sum
+//              of the j it meets next; no LDS atomics or cross-lane reductions.
 //              A pair closer than S + h_min (touching, or in the band where the
-//              reference clamps the vdW gap) contributes nothing to the sweep; it is
-//              recorded as a bit of a per-lane mask, and after the sweep each lane
-//              evaluates its near pairs once with the reference arithmetic and
-//              files the result (partner, force, torque, energy) in both balls'
-//              near-entry lists.  The SPLIT sweeps of a tile are combined in LDS
-//              in a fixed order, and each tile writes its i-side and j-side force
-//              sums once, to part[J][.][i] and part[I][.][j].
-//   k_ball     eight lanes per ball: sum the tile partials (fixed order; Kahan in
-//              the mixed mode), sort the ball's near entries by partner index and
-//              sum them in that order, then one lane forms acc and aacc, applies
-//              the closing kick, the write-step energies and the next step's
-//              drift (writing the other state set).
+//              reference clamps the vdW gap) contributes nothing to the sweep: it is
+//              only recorded, as a bit of the partner masks of both balls.  Each
+//              tile writes its i-side and j-side force sums and partner masks once,
+//              to slot J of the balls of block I and slot I of the balls of block J
+//              (system-local block numbers), so every (ball, slot) is written by
+//              exactly one tile per step and nothing needs clearing.
+//              After the sweep, wavefront 0 of the tile evaluates the tile's near
+//              pairs with the reference arithmetic, once each, and writes their sums
+//              for both balls to the same slots (near records, read only where the
+//              partner mask is nonzero), so the near-pair latency overlaps other
+//              tiles' far-field work.
+//   k_ball     one workgroup per 64-ball block, eight wavefronts with lane l
+//              working for ball l: sum the ball's slots in a fixed order (far field
+//              Kahan-compensated in the mixed mode, plus the near records); then
+//              wavefront 0 forms acc and aacc, applies the closing kick, the
+//              write-step energies and the next step's drift (writing the other
+//              state set).
 // Every sum has a fixed order, so results are reproducible run to run.
 #pragma once
 
 #include "sl_fast_pair.hpp"
 #include "sl_internal.hpp"
 #include "sl_pair_ref.hpp"
+
+#include <type_traits>
 
 #pragma clang fp contract(off)
 
@@ -37,7 +44,6 @@ namespace fastk {
 constexpr int TB = FAST_TILE;
 constexpr int NW = 4;                 // wavefronts per block
 constexpr int NT = TB * NW;
-constexpr int MAXNB = FAST_MAX_NEAR;
 
 // Lane l receives the value held by lane (l + 1) mod 64 (DPP wave_rol:1).
 __device__ __forceinline__ float rot1(float v)
@@ -66,6 +72,11 @@ __device__ __forceinline__ double rotk(double v, int k)
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
+__device__ __forceinline__ unsigned long long rotl64(unsigned long long x, int k)
+{
+    return k ? (x << k) | (x >> (64 - k)) : x;
+}
+
 template <int N>
 __device__ __forceinline__ double block_sum(double v, double* red)
 {
@@ -87,82 +98,77 @@ struct TileArgs {
     const double* pz;
     const double* R;
     FastAux aux;
-    State cur;              // full state, read by the near-pair evaluation
     const Phys* sysP;       // per system
+    const double4* bnd;     // per 64-ball block: bounding box of the centres (lo with max R, hi)
     int n_pad;
     int n_tiles;
     const int4* tiles;      // {I, J, system, first block of the system}
-    double L0inv2;          // 1 / L0^2 (mixed)
-    double Kv, Kg;          // mixed: force = Kv * vdw_sum + Kg * grav_sum
-    double* part;
-    int* nb_count;
-    NearEntry* nb_ent;
-    int* flags;
-    double* pe_tile;
+    float L0inv, L0inv2;    // mixed: 1 / L0 and 1 / L0^2 (scaled units)
+    double Kf;              // mixed: force = Kf * (scaled binary32 factor) * (r / L0)
+    double* part;           // [slot][xyz][n_pad]
+    unsigned long long* nbm;   // [slot][n_pad]: near-partner masks
+    double* nd;             // [slot][F xyz, T xyz][n_pad]: near sums, where the mask is nonzero
+    double* pe_tile;        // per tile: pair energy (write steps)
+    int* tile_nc;           // per tile: touching pairs (write steps)
+    unsigned long long* tile_ctr;   // per tile: running sums of touching and near pairs
+    State cur;              // full state, read by the near-pair evaluation
+    const int* perm;        // engine slot -> the caller's ball index (fixes pair orientation)
 };
-
-struct NearStateG {
-    const State& s;
-    int A, B;
-    __device__ D3 vA() const { return d3(s.f[VX][A], s.f[VY][A], s.f[VZ][A]); }
-    __device__ D3 vB() const { return d3(s.f[VX][B], s.f[VY][B], s.f[VZ][B]); }
-    __device__ D3 wA() const { return d3(s.f[WX][A], s.f[WY][A], s.f[WZ][A]); }
-    __device__ D3 wB() const { return d3(s.f[WX][B], s.f[WY][B], s.f[WZ][B]); }
-    __device__ double old_dist() const
-    {
-        return norm(d3(s.f[QX][B], s.f[QY][B], s.f[QZ][B]) - d3(s.f[QX][A], s.f[QY][A], s.f[QZ][A]));
-    }
-};
-
-__device__ __forceinline__ void file_entry(const TileArgs& a, int self, int other, int contact,
-                                           D3 F, D3 T, double pe)
-{
-    const int k = atomicAdd(&a.nb_count[self], 1);
-    if (k < MAXNB) {
-        NearEntry e;
-        e.j = other;
-        e.contact = contact;
-        e.fx = F.x; e.fy = F.y; e.fz = F.z;
-        e.tx = T.x; e.ty = T.y; e.tz = T.z;
-        e.pe = pe;
-        a.nb_ent[(size_t)self * MAXNB + k] = e;
-    } else {
-        atomicOr(&a.flags[0], 1);
-    }
-}
-
-// Near pairs of this lane (bit s of mask: the pair met at step s): evaluate each
-// once, in the reference's orientation (A = max index) and arithmetic, and file
-// it with both balls.  The pair energy goes with A only, so it is counted once.
-__device__ void eval_near(const TileArgs& a, const Phys& P, unsigned long long mask, int i, int J, int write)
-{
-    const int l = threadIdx.x % TB;
-    const State& c = a.cur;
-    while (mask) {
-        const int s = __ffsll((long long)mask) - 1;
-        mask &= mask - 1;
-        const int j = J * TB + ((l + s) & (TB - 1));
-        const int A = i > j ? i : j, B = i > j ? j : i;
-        PairIn in;
-        in.pA = d3(c.f[PX][A], c.f[PY][A], c.f[PZ][A]);
-        in.pB = d3(c.f[PX][B], c.f[PY][B], c.f[PZ][B]);
-        in.RA = c.f[RAD][A];
-        in.RB = c.f[RAD][B];
-        in.mA = c.f[MASS][A];
-        in.mB = c.f[MASS][B];
-        PairOut o;
-        NearStateG ns{c, A, B};
-        pair_ref(P, write != 0, in, ns, o);
-        file_entry(a, A, B, o.contact, o.F, o.tA, o.pe);
-        file_entry(a, B, A, o.contact ? 2 : 0, -o.F, o.tB, 0.0);
-    }
-}
 
 // Sum over the 64 lanes of a wavefront as a fixed tree (deterministic).
 __device__ __forceinline__ double wave_sum(double v)
 {
     for (int m = 1; m < TB; m <<= 1) v = v + __shfl_xor(v, m, TB);
     return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_or(unsigned long long v)
+{
+    for (int m = 1; m < TB; m <<= 1) v |= __shfl_xor(v, m, TB);
+    return v;
+}
+
+__device__ __forceinline__ int wave_isum(int v)
+{
+    for (int m = 1; m < TB; m <<= 1) v += __shfl_xor(v, m, TB);
+    return v;
+}
+
+// Bounding box of the valid balls held one per lane (every lane gets the result):
+// lo = (min x, min y, min z, max R), hi = (max x, max y, max z, 0).
+__device__ __forceinline__ void wave_aabb(double x, double y, double z, double R, bool valid, double4& lo,
+                                          double4& hi)
+{
+    const double inf = __builtin_huge_val();
+    double lx = valid ? x : inf, ly = valid ? y : inf, lz = valid ? z : inf;
+    double hx = valid ? x : -inf, hy = valid ? y : -inf, hz = valid ? z : -inf;
+    double rm = valid ? R : 0.0;
+    for (int m = 1; m < TB; m <<= 1) {
+        lx = fmin(lx, __shfl_xor(lx, m, TB));
+        ly = fmin(ly, __shfl_xor(ly, m, TB));
+        lz = fmin(lz, __shfl_xor(lz, m, TB));
+        hx = fmax(hx, __shfl_xor(hx, m, TB));
+        hy = fmax(hy, __shfl_xor(hy, m, TB));
+        hz = fmax(hz, __shfl_xor(hz, m, TB));
+        rm = fmax(rm, __shfl_xor(rm, m, TB));
+    }
+    lo = make_double4(lx, ly, lz, rm);
+    hi = make_double4(hx, hy, hz, 0.0);
+}
+
+// A tile is far when the gap between its two blocks' bounding boxes exceeds the two
+// largest radii plus h_min (with a 1e-9 relative margin): every pair in it is then
+// farther apart than S + h_min by far more than the rounding of the per-pair test,
+// so the sweep needs no near test.  Diagonal tiles always get the test.
+__device__ __forceinline__ bool tile_is_far(const TileArgs& a, int I, int J, double h_min)
+{
+    if (I == J) return false;
+    const double4 li = a.bnd[2 * I], ui = a.bnd[2 * I + 1], lj = a.bnd[2 * J], uj = a.bnd[2 * J + 1];
+    const double gx = fmax(0.0, fmax(lj.x - ui.x, li.x - uj.x));
+    const double gy = fmax(0.0, fmax(lj.y - ui.y, li.y - uj.y));
+    const double gz = fmax(0.0, fmax(lj.z - ui.z, li.z - uj.z));
+    const double lim = (li.w + lj.w + h_min) * (1 + 1e-9);
+    return gx * gx + gy * gy + gz * gz > lim * lim;
 }
 
 // vdW energy of a far pair from s = d^2 - S^2, t = d^2 - (Ri-Rj)^2.
@@ -187,117 +193,218 @@ __device__ __forceinline__ void sweep_range(bool diag, int h, int& s0, int& s1)
     }
 }
 
+__device__ __forceinline__ int4 fetch_tile(const TileArgs& a, int t, bool live)
+{
+    return live ? a.tiles[t] : make_int4(0, 0, 0, 0);
+}
+
 struct JRec64 {
     double x, y, z, R, q, g;
 };
 
-template <bool WRITE, bool GRAV, int SPLIT>
-__global__ __launch_bounds__(NT) void k_tiles_f64(TileArgs a)
+// The pair sweep of every tile: the far-field force of each pair, with the per-pair
+// near test only in tiles the bounding-box test cannot rule out.  A near pair
+// (closer than S + h_min) contributes nothing here; it is recorded as a bit of the
+// partner masks of both balls, for k_near.  SL_PREC_MIXED evaluates the factor in
+// binary32 from binary64 geometry, in units of L0 (median radius) and M0 (median
+// mass), as one sum for vdW and gravity: f = Kf (q_i q_j / (s t)^2 + g_i g_j / d^3)
+// with per-ball weights q = sqrt(Kv/Kf) (R/L0)^3, g = sqrt(Kg/Kf) m/M0, accumulated in
+// binary32 per tile (inside a dust aggregate the gravity term is some 1e-20 of the vdW
+// term, below the resolution of either format).  Each tile writes its i-side and
+// j-side sums and masks once, to slot J of the balls of block I and slot I of the
+// balls of block J, and its far-pair energy on write steps.
+template <int PREC, bool WRITE, bool GRAV, int SPLIT>
+__global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
 {
-    constexpr int TPB = NW / SPLIT;   // tiles per block
-    __shared__ JRec64 sj[TPB][2 * TB];
-    __shared__ double comb[NW][6][TB];
+    constexpr int TPB = NW / SPLIT;
+    constexpr size_t SJ_BYTES = sizeof(JRec64) * TPB * 2 * TB;
+    constexpr size_t COMB_BYTES = sizeof(double) * NW * 6 * TB;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[SJ_BYTES > COMB_BYTES ? SJ_BYTES : COMB_BYTES];
+    JRec64 (*sj)[2 * TB] = reinterpret_cast<JRec64 (*)[2 * TB]>(lds);
+    double (*comb)[6][TB] = reinterpret_cast<double (*)[6][TB]>(lds);
     __shared__ double pe_w[NW];
+    __shared__ unsigned long long im[TPB][TB], jm[TPB][TB];
 
     const int w = threadIdx.x / TB, l = threadIdx.x % TB;
     const int tl = w / SPLIT, h = w % SPLIT;
     const int t = blockIdx.x * TPB + tl;
     const bool live = t < a.n_tiles;
-    int I = 0, J = 0, sys = 0, blk0 = 0;
-    if (live) {
-        const int4 tt = a.tiles[t];
-        I = tt.x;
-        J = tt.y;
-        sys = tt.z;
-        blk0 = tt.w;
-    }
+    const int4 tt = fetch_tile(a, t, live);
+    const int I = tt.x, J = tt.y, sys = tt.z, blk0 = tt.w;
     const Phys& P = a.sysP[sys];
+    const bool diag = I == J;
+    const bool test = live && !tile_is_far(a, I, J, P.h_min);
     const int i = I * TB + l;
     const int j0 = J * TB + l;
     const double xi = a.px[i], yi = a.py[i], zi = a.pz[i], Ri = a.R[i];
-    const double qi = a.aux.q[i], gi = a.aux.g[i];
-    const double Ri4 = 4 * Ri;
-    if (h == 0) {
+    const double qi = PREC == SL_PREC_MIXED ? (double)a.aux.q32[i] : a.aux.q[i];
+    const double gi = PREC == SL_PREC_MIXED ? (double)a.aux.g32[i] : a.aux.g[i];
+    if (live && h == 0) {
         JRec64 r;
         r.x = a.px[j0];
         r.y = a.py[j0];
         r.z = a.pz[j0];
         r.R = a.R[j0];
-        r.q = a.aux.q[j0];
-        r.g = a.aux.g[j0];
+        r.q = PREC == SL_PREC_MIXED ? (double)a.aux.q32[j0] : a.aux.q[j0];
+        r.g = PREC == SL_PREC_MIXED ? (double)a.aux.g32[j0] : a.aux.g[j0];
         sj[tl][l] = r;
         sj[tl][l + TB] = r;
+        im[tl][l] = 0;
+        jm[tl][l] = 0;
     }
     __syncthreads();
 
-    const double h_min = P.h_min;
-    const double hmin2 = h_min * h_min;
-    const double two_hmin = 2 * h_min;
-    double ax = 0, ay = 0, az = 0;      // force on i
-    double bx = 0, by = 0, bz = 0;      // reaction on the j this lane meets next
-    double pe = 0;
-    const bool diag = I == J;
-    const JRec64* rec = &sj[tl][l];
+    double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0, pe = 0;
     unsigned long long nmask = 0;       // bit s: near pair met at step s
-
-    auto pair = [&](int s, bool act) {
-        const JRec64& r = rec[s];
-        const double dx = r.x - xi;
-        const double dy = r.y - yi;
-        const double dz = r.z - zi;
-        const double Rj = r.R;
-        const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-        const double S = Ri + Rj;
-        const double sv = __builtin_fma(-S, S, d2);          // d^2 - S^2
-        const double thr = __builtin_fma(S, two_hmin, hmin2); // (S + h_min)^2 - S^2
-        const double tv = __builtin_fma(Ri4, Rj, sv);        // d^2 - (Ri - Rj)^2
-        const double st = sv * tv;
-        double f = (qi * r.q) * rcp_f64(st * st);
-        double rinv = 0;
-        if (GRAV) {
-            rinv = rsq_f64(d2);
-            f = __builtin_fma(gi * r.g, rinv * rinv * rinv, f);
-        }
-        const bool near = sv < thr;
-        const bool use = act && !near;
-        f = use ? f : 0.0;
-        ax = __builtin_fma(f, dx, ax);
-        ay = __builtin_fma(f, dy, ay);
-        az = __builtin_fma(f, dz, az);
-        bx = rot1(__builtin_fma(-f, dx, bx));
-        by = rot1(__builtin_fma(-f, dy, by));
-        bz = rot1(__builtin_fma(-f, dz, bz));
-        if (WRITE && use) {
-            double u = far_energy(P.Ha, Ri, Rj, sv, tv);
-            if (GRAV) u = u - (gi * r.g) * rinv;
-            pe = pe + u;
-        }
-        nmask |= (act && near) ? (1ull << s) : 0ull;
-    };
-
     if (live) {
+        const JRec64* rec = &sj[tl][l];
+        const double Ri4 = 4 * Ri;
+        const double h_min = P.h_min;
+        const double hmin2 = h_min * h_min, two_hmin = 2 * h_min;
         int s0, s1;
         sweep_range<SPLIT>(diag, h, s0, s1);
-        if (!diag) {
-#pragma unroll (WRITE ? 1 : 8)
-            for (int s = s0; s < s1; s++) pair(s, true);
+        if constexpr (PREC == SL_PREC_MIXED) {
+            const float qif = (float)qi, gif = (float)gi, L0inv2 = a.L0inv2, L0inv = a.L0inv;
+            const float r4i = 4.0f * (float)Ri * L0inv;
+            float fax = 0.f, fay = 0.f, faz = 0.f, fbx = 0.f, fby = 0.f, fbz = 0.f;
+            auto pair = [&](int s, bool act, auto ntest) {
+                const JRec64& r = rec[s];
+                const double dx = r.x - xi, dy = r.y - yi, dz = r.z - zi;
+                const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                const double S = Ri + r.R;
+                const double sv = __builtin_fma(-S, S, d2);
+                const float s32 = (float)sv * L0inv2;
+                const float t32 = __builtin_fmaf(r4i, (float)r.R * L0inv, s32);
+                const float st = s32 * t32;
+                float f = (qif * (float)r.q) * __builtin_amdgcn_rcpf(st * st);
+                if (GRAV) {
+                    const float rq = __builtin_amdgcn_rsqf((float)d2 * L0inv2);
+                    f = __builtin_fmaf(gif * (float)r.g, rq * rq * rq, f);
+                }
+                bool use = true;
+                if constexpr (decltype(ntest)::value) {
+                    const bool near = sv < __builtin_fma(S, two_hmin, hmin2);
+                    use = act && !near;
+                    f = use ? f : 0.f;
+                    nmask |= (act && near) ? (1ull << s) : 0ull;
+                }
+                const float dxf = (float)dx * L0inv, dyf = (float)dy * L0inv, dzf = (float)dz * L0inv;
+                fax = __builtin_fmaf(f, dxf, fax);
+                fay = __builtin_fmaf(f, dyf, fay);
+                faz = __builtin_fmaf(f, dzf, faz);
+                fbx = rot1(__builtin_fmaf(-f, dxf, fbx));
+                fby = rot1(__builtin_fmaf(-f, dyf, fby));
+                fbz = rot1(__builtin_fmaf(-f, dzf, fbz));
+                if (WRITE && use) {
+                    const double tv = __builtin_fma(Ri4, r.R, sv);
+                    double u = far_energy(P.Ha, Ri, r.R, sv, tv);
+                    if (GRAV) u = u - (a.aux.g[i] * a.aux.g[J * TB + ((l + s) & (TB - 1))]) * rsq_f64(d2);
+                    pe = pe + u;
+                }
+            };
+            auto sweep = [&](auto ntest) {
+                if (!diag) {
+                    constexpr int NS = TB / SPLIT;
+                    if constexpr (WRITE) {
+                        for (int k = 0; k < NS; k++) pair(s0 + k, true, ntest);
+                    } else {
+#pragma unroll 8
+                        for (int k = 0; k < NS; k++) pair(s0 + k, true, ntest);
+                    }
+                } else {
+                    for (int s = s0; s < s1 - 1; s++) pair(s, true, ntest);
+                    pair(s1 - 1, s1 - 1 < TB / 2 || l < TB / 2, ntest);
+                }
+            };
+            if (test)
+                sweep(std::true_type{});
+            else
+                sweep(std::false_type{});
+            const int back = (TB - (s1 & (TB - 1))) & (TB - 1);
+            if (back) {
+                fbx = rotk(fbx, back);
+                fby = rotk(fby, back);
+                fbz = rotk(fbz, back);
+            }
+            const double K = a.Kf;
+            ax = K * (double)fax; ay = K * (double)fay; az = K * (double)faz;
+            bx = K * (double)fbx; by = K * (double)fby; bz = K * (double)fbz;
         } else {
-#pragma unroll (WRITE ? 1 : 4)
-            for (int s = s0; s < s1 - 1; s++) pair(s, true);
-            pair(s1 - 1, s1 - 1 < TB / 2 || l < TB / 2);
+            auto pair = [&](int s, bool act, auto ntest) {
+                const JRec64& r = rec[s];
+                const double dx = r.x - xi, dy = r.y - yi, dz = r.z - zi;
+                const double Rj = r.R;
+                const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                const double S = Ri + Rj;
+                const double sv = __builtin_fma(-S, S, d2);      // d^2 - S^2
+                const double tv = __builtin_fma(Ri4, Rj, sv);    // d^2 - (Ri - Rj)^2
+                const double st = sv * tv;
+                double f = (qi * r.q) * rcp_f64(st * st);
+                double rinv = 0;
+                if (GRAV) {
+                    rinv = rsq_f64(d2);
+                    f = __builtin_fma(gi * r.g, rinv * rinv * rinv, f);
+                }
+                bool use = true;
+                if constexpr (decltype(ntest)::value) {
+                    const bool near = sv < __builtin_fma(S, two_hmin, hmin2);   // (S + h_min)^2 - S^2
+                    use = act && !near;
+                    f = use ? f : 0.0;
+                    nmask |= (act && near) ? (1ull << s) : 0ull;
+                }
+                ax = __builtin_fma(f, dx, ax);
+                ay = __builtin_fma(f, dy, ay);
+                az = __builtin_fma(f, dz, az);
+                bx = rot1(__builtin_fma(-f, dx, bx));
+                by = rot1(__builtin_fma(-f, dy, by));
+                bz = rot1(__builtin_fma(-f, dz, bz));
+                if (WRITE && use) {
+                    double u = far_energy(P.Ha, Ri, Rj, sv, tv);
+                    if (GRAV) u = u - (gi * r.g) * rinv;
+                    pe = pe + u;
+                }
+            };
+            auto sweep = [&](auto ntest) {
+                if (!diag) {
+                    constexpr int NS = TB / SPLIT;
+                    if constexpr (WRITE) {
+                        for (int k = 0; k < NS; k++) pair(s0 + k, true, ntest);
+                    } else {
+#pragma unroll 8
+                        for (int k = 0; k < NS; k++) pair(s0 + k, true, ntest);
+                    }
+                } else {
+                    for (int s = s0; s < s1 - 1; s++) pair(s, true, ntest);
+                    pair(s1 - 1, s1 - 1 < TB / 2 || l < TB / 2, ntest);
+                }
+            };
+            if (test)
+                sweep(std::true_type{});
+            else
+                sweep(std::false_type{});
+            const int back = (TB - (s1 & (TB - 1))) & (TB - 1);
+            if (back) {
+                bx = rotk(bx, back);
+                by = rotk(by, back);
+                bz = rotk(bz, back);
+            }
         }
-        // lane l holds the reaction sum of slot l + s1: bring slot l home
-        const int back = (TB - (s1 & (TB - 1))) & (TB - 1);
-        if (back) {
-            bx = rotk(bx, back);
-            by = rotk(by, back);
-            bz = rotk(bz, back);
+        if (nmask) {
+            atomicOr(&im[tl][l], rotl64(nmask, l));
+            unsigned long long m = nmask;
+            while (m) {
+                const int s = __ffsll((long long)m) - 1;
+                m &= m - 1;
+                atomicOr(&jm[tl][(l + s) & (TB - 1)], 1ull << l);
+            }
         }
-        if (__ballot(nmask != 0)) eval_near(a, P, nmask, i, J, WRITE ? 1 : 0);
         if (WRITE) {
             const double ws = wave_sum(pe);
             if (l == 0) pe_w[w] = ws;
         }
+    }
+    __syncthreads();
+    if (live) {
         comb[w][0][l] = ax;
         comb[w][1][l] = ay;
         comb[w][2][l] = az;
@@ -306,206 +413,217 @@ __global__ __launch_bounds__(NT) void k_tiles_f64(TileArgs a)
         comb[w][5][l] = bz;
     }
     __syncthreads();
-    if (live && h == 0) {
-        // the SPLIT sweeps of this tile, in a fixed order
-        const int w0 = tl * SPLIT;
-        double v[6];
-        for (int k = 0; k < 6; k++) {
-            double acc = comb[w0][k][l];
-            for (int q = 1; q < SPLIT; q++) acc = acc + comb[w0 + q][k][l];
-            v[k] = acc;
-        }
-        const size_t np = (size_t)a.n_pad;
-        if (diag) {
-            // the i side and the j side of the diagonal tile land on the same balls
-            a.part[((size_t)(J - blk0) * 3 + 0) * np + i] = v[0] + v[3];
-            a.part[((size_t)(J - blk0) * 3 + 1) * np + i] = v[1] + v[4];
-            a.part[((size_t)(J - blk0) * 3 + 2) * np + i] = v[2] + v[5];
-        } else {
-            a.part[((size_t)(J - blk0) * 3 + 0) * np + i] = v[0];
-            a.part[((size_t)(J - blk0) * 3 + 1) * np + i] = v[1];
-            a.part[((size_t)(J - blk0) * 3 + 2) * np + i] = v[2];
-            a.part[((size_t)(I - blk0) * 3 + 0) * np + j0] = v[3];
-            a.part[((size_t)(I - blk0) * 3 + 1) * np + j0] = v[4];
-            a.part[((size_t)(I - blk0) * 3 + 2) * np + j0] = v[5];
-        }
+    if (!(live && h == 0)) return;
+    const int w0 = tl * SPLIT;
+    double v[6];
+    for (int k = 0; k < 6; k++) {
+        double acc = comb[w0][k][l];
+        for (int q = 1; q < SPLIT; q++) acc = acc + comb[w0 + q][k][l];
+        v[k] = acc;
     }
-    if (WRITE && live && h == 0 && l == 0) {
-        double pt = pe_w[tl * SPLIT];
-        for (int q = 1; q < SPLIT; q++) pt = pt + pe_w[tl * SPLIT + q];
+    const size_t np = (size_t)a.n_pad;
+    const size_t si = (size_t)(J - blk0), sj_ = (size_t)(I - blk0);
+    const unsigned long long mi = im[tl][l], mj = jm[tl][l];
+    if (diag) {
+        // the i side and the j side of the diagonal tile land on the same balls
+        a.part[(si * 3 + 0) * np + i] = v[0] + v[3];
+        a.part[(si * 3 + 1) * np + i] = v[1] + v[4];
+        a.part[(si * 3 + 2) * np + i] = v[2] + v[5];
+        a.nbm[si * np + i] = mi | mj;
+    } else {
+        a.part[(si * 3 + 0) * np + i] = v[0];
+        a.part[(si * 3 + 1) * np + i] = v[1];
+        a.part[(si * 3 + 2) * np + i] = v[2];
+        a.part[(sj_ * 3 + 0) * np + j0] = v[3];
+        a.part[(sj_ * 3 + 1) * np + j0] = v[4];
+        a.part[(sj_ * 3 + 2) * np + j0] = v[5];
+        a.nbm[si * np + i] = mi;
+        a.nbm[sj_ * np + j0] = mj;
+    }
+    if (WRITE && l == 0) {
+        double pt = pe_w[w0];
+        for (int q = 1; q < SPLIT; q++) pt = pt + pe_w[w0 + q];
         a.pe_tile[t] = pt;
     }
 }
 
-struct JRec32 {
-    double x, y, z, R;
-    float r, q, g, pad;
+// Velocities, spins and previous positions of a pair's two balls, read only when
+// the pair touches.
+struct NearStateG {
+    const State& s;
+    int A, B;
+    __device__ D3 vA() const { return d3(s.f[VX][A], s.f[VY][A], s.f[VZ][A]); }
+    __device__ D3 vB() const { return d3(s.f[VX][B], s.f[VY][B], s.f[VZ][B]); }
+    __device__ D3 wA() const { return d3(s.f[WX][A], s.f[WY][A], s.f[WZ][A]); }
+    __device__ D3 wB() const { return d3(s.f[WX][B], s.f[WY][B], s.f[WZ][B]); }
+    __device__ double old_dist() const
+    {
+        return norm(d3(s.f[QX][B], s.f[QY][B], s.f[QZ][B]) - d3(s.f[QX][A], s.f[QY][A], s.f[QZ][A]));
+    }
 };
 
-// Mixed precision: relative displacement, d^2, the near test and s = d^2 - S^2
-// in binary64; the vdW and gravity factors and the per-tile force sums in
-// binary32 (scaled units; separate sums for vdW and gravity, whose ratio can
-// reach 1e-20); tile sums converted to binary64 forces when combined.
-template <bool WRITE, bool GRAV, int SPLIT>
-__global__ __launch_bounds__(NT) void k_tiles_mixed(TileArgs a)
+__device__ __forceinline__ D3 load3(const State& c, int f, int b)
 {
-    constexpr int TPB = NW / SPLIT;
-    __shared__ JRec32 sj[TPB][2 * TB];
-    __shared__ double comb[NW][6][TB];
-    __shared__ double pe_w[NW];
+    return d3(c.f[f][b], c.f[f + 1][b], c.f[f + 2][b]);
+}
 
-    const int w = threadIdx.x / TB, l = threadIdx.x % TB;
-    const int tl = w / SPLIT, h = w % SPLIT;
-    const int t = blockIdx.x * TPB + tl;
-    const bool live = t < a.n_tiles;
-    int I = 0, J = 0, sys = 0, blk0 = 0;
-    if (live) {
-        const int4 tt = a.tiles[t];
-        I = tt.x;
-        J = tt.y;
-        sys = tt.z;
-        blk0 = tt.w;
+struct NearAcc {
+    double F[3], T[3], pe;
+    int contacts, pairs;
+};
+
+// Result of one (ball, partner) near evaluation, for the ball: force, torque, and the
+// pair's energy and touching flag when the ball is A (counted once per pair).
+struct NearRes {
+    double f[6];
+    double pe;
+    int contact, is_a;
+};
+
+constexpr int NEAR_CH = NT;   // near evaluations per workgroup round (one per thread)
+
+// Evaluate the near pair of ball b and partner p with the reference arithmetic in the
+// reference orientation: A is the ball with the larger index in the caller's
+// numbering (the friction terms are not symmetric in A and B).
+template <bool WRITE>
+__device__ __forceinline__ void eval_entry(const State& c, const int* perm, const Phys& P, int b, int p, NearRes& r)
+{
+    const bool is_a = perm[b] > perm[p];
+    const int A = is_a ? b : p, B = is_a ? p : b;
+    PairIn in;
+    in.pA = load3(c, PX, A);
+    in.pB = load3(c, PX, B);
+    in.RA = c.f[RAD][A];
+    in.RB = c.f[RAD][B];
+    in.mA = c.f[MASS][A];
+    in.mB = c.f[MASS][B];
+    NearStateG ns{c, A, B};
+    PairOut po;
+    pair_ref(P, WRITE, in, ns, po);
+    const D3 F = is_a ? po.F : -po.F;
+    const D3 T = is_a ? po.tA : po.tB;
+    r.f[0] = F.x; r.f[1] = F.y; r.f[2] = F.z;
+    r.f[3] = T.x; r.f[4] = T.y; r.f[5] = T.z;
+    r.pe = is_a ? po.pe : 0.0;
+    r.contact = is_a ? po.contact : 0;
+    r.is_a = is_a;
+}
+
+// The near pairs of every tile whose masks are not all zero (one workgroup per tile;
+// the others exit at once).  Lane l of wavefront 0 owns the tile's i ball 64 I + l and
+// j ball 64 J + l; its entries are the partners of its i ball (in block J; on the
+// diagonal, all of its partners), then those of its j ball (in block I): every near
+// pair is evaluated by the lanes of both of its balls, identically.  The entries of
+// all lanes are numbered in lane order and evaluated NEAR_CH at a time, one per
+// thread of the workgroup; each lane of wavefront 0 then adds its own entries' results
+// in entry order and writes the near sums of its balls to their slots.
+template <bool WRITE>
+__global__ __launch_bounds__(NT) void k_near(TileArgs a)
+{
+    __shared__ NearRes res[NEAR_CH];
+    __shared__ int ent[NEAR_CH];
+    __shared__ int total_s;
+    const int tid = threadIdx.x, l = tid % TB;
+    const int t = blockIdx.x;
+    const int4 tt = a.tiles[t];
+    const int I = tt.x, J = tt.y, sys = tt.z, blk0 = tt.w;
+    const bool diag = I == J;
+    const size_t np = (size_t)a.n_pad;
+    const size_t si = (size_t)(J - blk0), sj_ = (size_t)(I - blk0);
+    const int i = I * TB + l, j0 = J * TB + l;
+    unsigned long long ma = 0, mb = 0;
+    int ca = 0, cnt = 0, base = 0;
+    if (tid < TB) {
+        ma = a.nbm[si * np + i];
+        mb = diag ? 0ull : a.nbm[sj_ * np + j0];
+        ca = __popcll(ma);
+        cnt = ca + __popcll(mb);
+        int incl = cnt;
+        for (int m = 1; m < TB; m <<= 1) {
+            const int y = __shfl_up(incl, m, TB);
+            if (l >= m) incl += y;
+        }
+        base = incl - cnt;
+        const int tot = __shfl(incl, TB - 1, TB);
+        if (tid == 0) total_s = tot;
+    }
+    __syncthreads();
+    const int total = total_s;
+    if (total == 0) {
+        if (WRITE && tid == 0) a.tile_nc[t] = 0;
+        return;   // uniform over the workgroup
     }
     const Phys& P = a.sysP[sys];
-    const int i = I * TB + l;
-    const int j0 = J * TB + l;
-    const double xi = a.px[i], yi = a.py[i], zi = a.pz[i], Ri = a.R[i];
-    const float r4i = 4.0f * a.aux.r32[i], qi = a.aux.q32[i], gi = a.aux.g32[i];
-    const double gi64 = a.aux.g[i];
-    if (h == 0) {
-        JRec32 r;
-        r.x = a.px[j0];
-        r.y = a.py[j0];
-        r.z = a.pz[j0];
-        r.R = a.R[j0];
-        r.r = a.aux.r32[j0];
-        r.q = a.aux.q32[j0];
-        r.g = a.aux.g32[j0];
-        r.pad = 0.f;
-        sj[tl][l] = r;
-        sj[tl][l + TB] = r;
-    }
-    __syncthreads();
-
-    const double h_min = P.h_min;
-    const double hmin2 = h_min * h_min;
-    const double two_hmin = 2 * h_min;
-    const double L0inv2 = a.L0inv2;
-    float vx = 0.f, vy = 0.f, vz = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;       // on i
-    float wvx = 0.f, wvy = 0.f, wvz = 0.f, wgx = 0.f, wgy = 0.f, wgz = 0.f; // travelling j sums
-    double pe = 0;
-    const bool diag = I == J;
-    const JRec32* rec = &sj[tl][l];
-    unsigned long long nmask = 0;
-
-    auto pair = [&](int s, bool act) {
-        const JRec32& r = rec[s];
-        const double dx = r.x - xi;
-        const double dy = r.y - yi;
-        const double dz = r.z - zi;
-        const double Rj = r.R;
-        const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-        const double S = Ri + Rj;
-        const double sv = __builtin_fma(-S, S, d2);
-        const double thr = __builtin_fma(S, two_hmin, hmin2);
-        const bool near = sv < thr;
-        const bool use = act && !near;
-        const float s32 = (float)(sv * L0inv2);
-        const float t32 = __builtin_fmaf(r4i, r.r, s32);
-        const float st = s32 * t32;
-        float fv = (qi * r.q) * __builtin_amdgcn_rcpf(st * st);
-        fv = use ? fv : 0.f;
-        const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
-        vx = __builtin_fmaf(fv, dxf, vx);
-        vy = __builtin_fmaf(fv, dyf, vy);
-        vz = __builtin_fmaf(fv, dzf, vz);
-        wvx = rot1(__builtin_fmaf(-fv, dxf, wvx));
-        wvy = rot1(__builtin_fmaf(-fv, dyf, wvy));
-        wvz = rot1(__builtin_fmaf(-fv, dzf, wvz));
-        if (GRAV) {
-            const float rif = __builtin_amdgcn_rsqf((float)d2);
-            float fg = (gi * r.g) * (rif * rif * rif);
-            fg = use ? fg : 0.f;
-            gx = __builtin_fmaf(fg, dxf, gx);
-            gy = __builtin_fmaf(fg, dyf, gy);
-            gz = __builtin_fmaf(fg, dzf, gz);
-            wgx = rot1(__builtin_fmaf(-fg, dxf, wgx));
-            wgy = rot1(__builtin_fmaf(-fg, dyf, wgy));
-            wgz = rot1(__builtin_fmaf(-fg, dzf, wgz));
-        }
-        if (WRITE && use) {
-            // write-step energy in binary64 (not on the per-step path)
-            const double tv = __builtin_fma(4 * Ri, Rj, sv);
-            double u = far_energy(P.Ha, Ri, Rj, sv, tv);
-            if (GRAV) u = u - (gi64 * a.aux.g[J * TB + ((l + s) & (TB - 1))]) * rsq_f64(d2);
-            pe = pe + u;
-        }
-        nmask |= (act && near) ? (1ull << s) : 0ull;
-    };
-
-    if (live) {
-        int s0, s1;
-        sweep_range<SPLIT>(diag, h, s0, s1);
-        if (!diag) {
-#pragma unroll (WRITE ? 1 : 8)
-            for (int s = s0; s < s1; s++) pair(s, true);
-        } else {
-#pragma unroll (WRITE ? 1 : 4)
-            for (int s = s0; s < s1 - 1; s++) pair(s, true);
-            pair(s1 - 1, s1 - 1 < TB / 2 || l < TB / 2);
-        }
-        const int back = (TB - (s1 & (TB - 1))) & (TB - 1);
-        if (back) {
-            wvx = rotk(wvx, back); wvy = rotk(wvy, back); wvz = rotk(wvz, back);
-            if (GRAV) {
-                wgx = rotk(wgx, back); wgy = rotk(wgy, back); wgz = rotk(wgz, back);
+    NearAcc ni = {{0, 0, 0}, {0, 0, 0}, 0, 0, 0}, nj = ni;
+    const State& c = a.cur;
+    const int Kp = diag ? I : J;
+    for (int c0 = 0; c0 < total; c0 += NEAR_CH) {
+        const int c1 = c0 + NEAR_CH;
+        if (tid < TB && cnt && base < c1 && base + cnt > c0) {
+            int k = base;
+            for (int side = 0; side < 2 && k < c1; side++) {
+                unsigned long long m = side ? mb : ma;
+                while (m && k < c1) {
+                    const int bit = __ffsll((long long)m) - 1;
+                    m &= m - 1;
+                    if (k >= c0) ent[k - c0] = l | (bit << 6) | (side << 12);
+                    k++;
+                }
             }
         }
-        if (__ballot(nmask != 0)) eval_near(a, P, nmask, i, J, WRITE ? 1 : 0);
-        if (WRITE) {
-            const double ws = wave_sum(pe);
-            if (l == 0) pe_w[w] = ws;
+        __syncthreads();
+        if (tid < total - c0) {
+            const int e = ent[tid];
+            const int el = e & (TB - 1), bit = (e >> 6) & (TB - 1), side = e >> 12;
+            const int b = side ? J * TB + el : I * TB + el;
+            const int p = side ? I * TB + bit : Kp * TB + bit;
+            NearRes r;
+            eval_entry<WRITE>(c, a.perm, P, b, p, r);
+            res[tid] = r;
         }
-        const double Kv = a.Kv, Kg = a.Kg;
-        double v[6] = {Kv * (double)vx, Kv * (double)vy, Kv * (double)vz,
-                       Kv * (double)wvx, Kv * (double)wvy, Kv * (double)wvz};
-        if (GRAV) {
-            v[0] = v[0] + Kg * (double)gx;
-            v[1] = v[1] + Kg * (double)gy;
-            v[2] = v[2] + Kg * (double)gz;
-            v[3] = v[3] + Kg * (double)wgx;
-            v[4] = v[4] + Kg * (double)wgy;
-            v[5] = v[5] + Kg * (double)wgz;
+        __syncthreads();
+        if (tid < TB && cnt && base < c1 && base + cnt > c0) {
+            const int k0 = base > c0 ? base : c0, k1 = base + cnt < c1 ? base + cnt : c1;
+            for (int k = k0; k < k1; k++) {
+                const NearRes& r = res[k - c0];
+                NearAcc& acc = (k - base < ca) ? ni : nj;
+                for (int q = 0; q < 3; q++) {
+                    acc.F[q] = acc.F[q] + r.f[q];
+                    acc.T[q] = acc.T[q] + r.f[3 + q];
+                }
+                acc.pe = acc.pe + r.pe;
+                acc.contacts += r.contact;
+                acc.pairs += r.is_a;
+            }
         }
-        for (int k = 0; k < 6; k++) comb[w][k][l] = v[k];
+        __syncthreads();
     }
-    __syncthreads();
-    if (live && h == 0) {
-        const int w0 = tl * SPLIT;
-        double v[6];
-        for (int k = 0; k < 6; k++) {
-            double acc = comb[w0][k][l];
-            for (int q = 1; q < SPLIT; q++) acc = acc + comb[w0 + q][k][l];
-            v[k] = acc;
-        }
-        const size_t np = (size_t)a.n_pad;
-        if (diag) {
-            a.part[((size_t)(J - blk0) * 3 + 0) * np + i] = v[0] + v[3];
-            a.part[((size_t)(J - blk0) * 3 + 1) * np + i] = v[1] + v[4];
-            a.part[((size_t)(J - blk0) * 3 + 2) * np + i] = v[2] + v[5];
-        } else {
-            a.part[((size_t)(J - blk0) * 3 + 0) * np + i] = v[0];
-            a.part[((size_t)(J - blk0) * 3 + 1) * np + i] = v[1];
-            a.part[((size_t)(J - blk0) * 3 + 2) * np + i] = v[2];
-            a.part[((size_t)(I - blk0) * 3 + 0) * np + j0] = v[3];
-            a.part[((size_t)(I - blk0) * 3 + 1) * np + j0] = v[4];
-            a.part[((size_t)(I - blk0) * 3 + 2) * np + j0] = v[5];
+    if (tid >= TB) return;
+    if (ma) {
+        double* d = a.nd + si * 6 * np + i;
+        for (int k = 0; k < 3; k++) {
+            d[k * np] = ni.F[k];
+            d[(3 + k) * np] = ni.T[k];
         }
     }
-    if (WRITE && live && h == 0 && l == 0) {
-        double pt = pe_w[tl * SPLIT];
-        for (int q = 1; q < SPLIT; q++) pt = pt + pe_w[tl * SPLIT + q];
-        a.pe_tile[t] = pt;
+    if (mb) {
+        double* d = a.nd + sj_ * 6 * np + j0;
+        for (int k = 0; k < 3; k++) {
+            d[k * np] = nj.F[k];
+            d[(3 + k) * np] = nj.T[k];
+        }
+    }
+    const int nc = wave_isum(ni.contacts + nj.contacts), nn = wave_isum(ni.pairs + nj.pairs);
+    if (WRITE) {
+        const double near_pe = wave_sum(ni.pe + nj.pe);
+        if (l == 0) {
+            a.pe_tile[t] = a.pe_tile[t] + near_pe;
+            a.tile_nc[t] = nc;
+        }
+    }
+    if (l == 0) {
+        a.tile_ctr[2 * t] += (unsigned long long)nc;
+        a.tile_ctr[2 * t + 1] += (unsigned long long)nn;
     }
 }
 
@@ -518,143 +636,149 @@ struct BallArgs {
     const int* sys_nb;
     int n_used, n_pad;
     const double* part;
-    int* nb_count;
-    const NearEntry* nb_ent;
-    int write, kick, drift_next, compensated;
-    double* e_blk;      // FAST_E_STRIDE doubles per 32-ball block (write steps)
+    const unsigned long long* nbm;
+    const double* nd;
+    int kick, drift_next;
+    double4* bnd;       // per 64-ball block: bounding box of the drifted positions (lo, hi)
+    double* e_blk;      // FAST_E_STRIDE doubles per 64-ball block (write steps)
     int* flags;         // [1]: a non-finite state appeared
-    unsigned long long* counters;
 };
 
-constexpr int LPB = 8;               // lanes per ball in k_ball
-constexpr int BPB = 256 / LPB;       // balls per 256-thread block
+constexpr int KB_W = 8;              // wavefronts per k_ball block (one 64-ball block)
+constexpr int KB_T = KB_W * TB;
+constexpr int BPB = TB;              // balls per k_ball block
+constexpr int SLOTS_IN_FLIGHT = 8;   // slots per lane loaded together
 
-// Sum over the LPB lanes of one ball as a fixed tree (deterministic).
-__device__ __forceinline__ double lane_group_sum(double v)
+// One 64-ball block per workgroup; lane l of every wavefront works for ball l.
+// Wavefront w sums the slots K = w, w + 8, ... in order: far-field partials (reads
+// of 64 consecutive doubles; Kahan-compensated in the mixed mode) and, where the
+// partner mask is nonzero, the tile's near-pair sums.  The eight wavefronts' sums
+// are combined in LDS in a fixed order and wavefront 0 finishes the step for the 64
+// balls (coalesced stores).
+template <bool COMP, bool WRITE>
+__global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
 {
-    v = v + __shfl_xor(v, 1, LPB);
-    v = v + __shfl_xor(v, 2, LPB);
-    v = v + __shfl_xor(v, 4, LPB);
-    return v;
-}
-
-__global__ __launch_bounds__(256) void k_ball(BallArgs a)
-{
-    __shared__ double red[256];
-    __shared__ int key[BPB][MAXNB];
-    __shared__ int order[BPB][MAXNB];
-    const int g = threadIdx.x % LPB;
-    const int bl = threadIdx.x / LPB;
-    const int i = blockIdx.x * BPB + bl;
-    const int sys = a.blk_sys[(i < a.n_used ? i : 0) / TB];
-    const bool valid = i < a.n_used && (i - a.sys_off[sys]) < a.sys_n[sys];
-    const int ic = valid ? i : 0;
-    const int nblk = a.sys_nb[sys];
+    __shared__ double sf[KB_W][9][TB];       // far force, near force, near torque
+    const int w = threadIdx.x / TB, l = threadIdx.x % TB;
+    const int i = blockIdx.x * TB + l;
+    const int sys = a.blk_sys[blockIdx.x];
+    const int off = a.sys_off[sys];
+    const bool valid = (i - off) < a.sys_n[sys];
+    const int ic = valid ? i : off;
+    const int nblk = valid ? a.sys_nb[sys] : 0;
+    const Phys& P = a.sysP[sys];
     const State& c = a.cur;
     const size_t np = (size_t)a.n_pad;
-    double e[FAST_E_STRIDE] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // PE KE p L contacts nonfinite
-    unsigned ncontact = 0;
 
-    // the ball's own state, loaded up front (the loads cannot cross the barriers below)
-    const double hx0 = c.f[HX][ic], hy0 = c.f[HY][ic], hz0 = c.f[HZ][ic];
-    const double gx0 = c.f[GX][ic], gy0 = c.f[GY][ic], gz0 = c.f[GZ][ic];
-    const double vx0 = c.f[VX][ic], vy0 = c.f[VY][ic], vz0 = c.f[VZ][ic];
-    const double wx0 = c.f[WX][ic], wy0 = c.f[WY][ic], wz0 = c.f[WZ][ic];
-    const double qx0 = c.f[QX][ic], qy0 = c.f[QY][ic], qz0 = c.f[QZ][ic];
-    const double px = c.f[PX][ic], py = c.f[PY][ic], pz = c.f[PZ][ic];
-    const double mi = c.f[MASS][ic], Ii = c.f[MOI][ic];
+    // wavefront 0 finishes the step: its own-state loads go out first
+    D3 p0 = d3(0, 0, 0), h0 = p0, g0 = p0;
+    double mi = 1, Ii = 1, Rb = 0;
+    if (w == 0 && valid) {
+        p0 = load3(c, PX, i);
+        h0 = load3(c, HX, i);
+        g0 = load3(c, GX, i);
+        mi = c.f[MASS][i];
+        Ii = c.f[MOI][i];
+        Rb = c.f[RAD][i];
+    }
 
-    // near entries: partner keys into LDS, order by partner index (rank = keys below)
-    const int cnt_raw = valid ? a.nb_count[ic] : 0;
-    const int cnt = cnt_raw < MAXNB ? cnt_raw : MAXNB;
-    const NearEntry* ent = a.nb_ent + (size_t)ic * MAXNB;
-    for (int u = g; u < cnt; u += LPB) key[bl][u] = ent[u].j;
-
-    // far field: lane g sums the tile partials K = g, g+8, ... of the ball's system
-    double fx = 0, fy = 0, fz = 0;
-    if (valid) {
-        if (a.compensated) {
-            double cx = 0, cy = 0, cz = 0;   // Kahan compensation
-#pragma unroll 4
-            for (int K = g; K < nblk; K += LPB) {
-                const double* p = a.part + (size_t)K * 3 * np + ic;
+    double fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
+    double n[6] = {0, 0, 0, 0, 0, 0};
+    for (int K0 = w; K0 < nblk; K0 += SLOTS_IN_FLIGHT * KB_W) {
+        double q[SLOTS_IN_FLIGHT][3];
+        unsigned long long mk[SLOTS_IN_FLIGHT];
+#pragma unroll
+        for (int u = 0; u < SLOTS_IN_FLIGHT; u++) {
+            const int K = K0 + u * KB_W;
+            const int Kc = K < nblk ? K : 0;
+            const double* pp = a.part + (size_t)Kc * 3 * np + ic;
+            q[u][0] = pp[0];
+            q[u][1] = pp[np];
+            q[u][2] = pp[2 * np];
+            mk[u] = K < nblk ? a.nbm[(size_t)Kc * np + ic] : 0ull;
+        }
+        unsigned nz = 0;
+#pragma unroll
+        for (int u = 0; u < SLOTS_IN_FLIGHT; u++) {
+            if (K0 + u * KB_W >= nblk) break;
+            if (mk[u]) nz |= 1u << u;
+            if (COMP) {
                 double y, t;
-                y = p[0] - cx; t = fx + y; cx = (t - fx) - y; fx = t;
-                y = p[np] - cy; t = fy + y; cy = (t - fy) - y; fy = t;
-                y = p[2 * np] - cz; t = fz + y; cz = (t - fz) - y; fz = t;
-            }
-            fx = fx - cx;
-            fy = fy - cy;
-            fz = fz - cz;
-        } else {
-#pragma unroll 4
-            for (int K = g; K < nblk; K += LPB) {
-                const double* p = a.part + (size_t)K * 3 * np + ic;
-                fx = fx + p[0];
-                fy = fy + p[np];
-                fz = fz + p[2 * np];
+                y = q[u][0] - cx; t = fx + y; cx = (t - fx) - y; fx = t;
+                y = q[u][1] - cy; t = fy + y; cy = (t - fy) - y; fy = t;
+                y = q[u][2] - cz; t = fz + y; cz = (t - fz) - y; fz = t;
+            } else {
+                fx = fx + q[u][0];
+                fy = fy + q[u][1];
+                fz = fz + q[u][2];
             }
         }
+        // near sums of the slots whose tile found near pairs for this ball
+        while (nz) {
+            const int u = __ffs(nz) - 1;
+            nz &= nz - 1;
+            const double* d = a.nd + (size_t)(K0 + u * KB_W) * 6 * np + ic;
+            for (int k = 0; k < 6; k++) n[k] = n[k] + d[k * np];
+        }
     }
+    sf[w][0][l] = fx - cx;
+    sf[w][1][l] = fy - cy;
+    sf[w][2][l] = fz - cz;
+    for (int k = 0; k < 6; k++) sf[w][3 + k][l] = n[k];
     __syncthreads();
-    for (int u = g; u < cnt; u += LPB) {
-        const int v = key[bl][u];
-        int below = 0;
-        for (int x = 0; x < cnt; x++) below += key[bl][x] < v;
-        order[bl][below] = u;
-    }
-    __syncthreads();
+    if (w != 0) return;
 
-    // near contributions in partner order: lane g takes ranks g, g+8, ...
-    double nfx = 0, nfy = 0, nfz = 0, ntx = 0, nty = 0, ntz = 0, pe = 0;
-    for (int r = g; r < cnt; r += LPB) {
-        const NearEntry en = ent[order[bl][r]];
-        nfx = nfx + en.fx;
-        nfy = nfy + en.fy;
-        nfz = nfz + en.fz;
-        ntx = ntx + en.tx;
-        nty = nty + en.ty;
-        ntz = ntz + en.tz;
-        pe = pe + en.pe;
-        ncontact += en.contact == 1;
+    // wavefront 0: combine in a fixed order and finish the step of ball l
+    double F[3], T[3];
+    for (int k = 0; k < 3; k++) {
+        double s = sf[0][k][l], cmp = 0;
+        for (int q = 1; q < KB_W; q++) {
+            if (COMP) {
+                const double y = sf[q][k][l] - cmp, t = s + y;
+                cmp = (t - s) - y;
+                s = t;
+            } else {
+                s = s + sf[q][k][l];
+            }
+        }
+        double nsum = sf[0][3 + k][l], tsum = sf[0][6 + k][l];
+        for (int q = 1; q < KB_W; q++) {
+            nsum = nsum + sf[q][3 + k][l];
+            tsum = tsum + sf[q][6 + k][l];
+        }
+        F[k] = (s - cmp) + nsum;
+        T[k] = tsum;
     }
-    fx = lane_group_sum(fx + nfx);
-    fy = lane_group_sum(fy + nfy);
-    fz = lane_group_sum(fz + nfz);
-    const double tx = lane_group_sum(ntx), ty = lane_group_sum(nty), tz = lane_group_sum(ntz);
-    pe = lane_group_sum(pe);
-    ncontact = (unsigned)lane_group_sum((double)ncontact);
-    unsigned nnear = 0;
-    if (valid && g == 0) {
-        nnear = (unsigned)cnt_raw;
-        a.nb_count[i] = 0;   // ready for the next step's tile pass
-        const double ax = fx / mi, ay = fy / mi, az = fz / mi;
-        const double bx = tx / Ii, by = ty / Ii, bz = tz / Ii;
-        const double dt = a.sysP[sys].dt;
+
+    double e[FAST_E_STRIDE] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // PE KE p L contacts nonfinite
+    D3 pn = p0;   // positions the next pair pass will read
+    if (valid) {
+        const double px = p0.x, py = p0.y, pz = p0.z;
+        const double ax = F[0] / mi, ay = F[1] / mi, az = F[2] / mi;
+        const double bx = T[0] / Ii, by = T[1] / Ii, bz = T[2] / Ii;
+        const double dt = P.dt;
         double vx, vy, vz, wx, wy, wz;
         if (a.kick) {
-            vx = hx0 + .5 * ax * dt;
-            vy = hy0 + .5 * ay * dt;
-            vz = hz0 + .5 * az * dt;
-            wx = gx0 + .5 * bx * dt;
-            wy = gy0 + .5 * by * dt;
-            wz = gz0 + .5 * bz * dt;
+            vx = h0.x + .5 * ax * dt;
+            vy = h0.y + .5 * ay * dt;
+            vz = h0.z + .5 * az * dt;
+            wx = g0.x + .5 * bx * dt;
+            wy = g0.y + .5 * by * dt;
+            wz = g0.z + .5 * bz * dt;
         } else {
-            vx = vx0; vy = vy0; vz = vz0;
-            wx = wx0; wy = wy0; wz = wz0;
+            vx = c.f[VX][i]; vy = c.f[VY][i]; vz = c.f[VZ][i];
+            wx = c.f[WX][i]; wy = c.f[WY][i]; wz = c.f[WZ][i];
         }
         const bool bad = !(isfinite(vx) && isfinite(vy) && isfinite(vz) && isfinite(wx) && isfinite(wy) &&
                            isfinite(wz) && isfinite(px) && isfinite(py) && isfinite(pz));
         if (bad) atomicOr(&a.flags[1], 1);
-        if (a.write) {
-            const D3 v = d3(vx, vy, vz), w = d3(wx, wy, wz), p = d3(px, py, pz);
-            e[0] = pe;
-            e[1] = .5 * mi * dot(v, v) + .5 * Ii * dot(w, w);
+        if (WRITE) {
+            const D3 v = d3(vx, vy, vz), wv = d3(wx, wy, wz);
+            e[1] = .5 * mi * dot(v, v) + .5 * Ii * dot(wv, wv);
             const D3 mv = v * mi;
             e[2] = mv.x; e[3] = mv.y; e[4] = mv.z;
-            const D3 L = cross(p, v) * mi + w * Ii;
+            const D3 L = cross(p0, v) * mi + wv * Ii;
             e[5] = L.x; e[6] = L.y; e[7] = L.z;
-            e[8] = (double)ncontact;
             e[9] = bad ? 1.0 : 0.0;
         }
         const State& o = a.nxt;
@@ -669,36 +793,50 @@ __global__ __launch_bounds__(256) void k_ball(BallArgs a)
             o.f[GY][i] = wy + .5 * by * dt;
             o.f[GZ][i] = wz + .5 * bz * dt;
             o.f[QX][i] = px; o.f[QY][i] = py; o.f[QZ][i] = pz;
-            o.f[PX][i] = px + hx * dt;
-            o.f[PY][i] = py + hy * dt;
-            o.f[PZ][i] = pz + hz * dt;
+            pn = d3(px + hx * dt, py + hy * dt, pz + hz * dt);
+            o.f[PX][i] = pn.x;
+            o.f[PY][i] = pn.y;
+            o.f[PZ][i] = pn.z;
         } else {
-            o.f[HX][i] = hx0; o.f[HY][i] = hy0; o.f[HZ][i] = hz0;
-            o.f[GX][i] = gx0; o.f[GY][i] = gy0; o.f[GZ][i] = gz0;
-            o.f[QX][i] = qx0; o.f[QY][i] = qy0; o.f[QZ][i] = qz0;
+            o.f[HX][i] = h0.x; o.f[HY][i] = h0.y; o.f[HZ][i] = h0.z;
+            o.f[GX][i] = g0.x; o.f[GY][i] = g0.y; o.f[GZ][i] = g0.z;
+            o.f[QX][i] = c.f[QX][i]; o.f[QY][i] = c.f[QY][i]; o.f[QZ][i] = c.f[QZ][i];
             o.f[PX][i] = px; o.f[PY][i] = py; o.f[PZ][i] = pz;
         }
     }
-    if (a.write) {
-        // a 32-ball block lies inside one 64-aligned system, so its sums are per system
-        for (int k = 0; k < FAST_E_STRIDE; k++) {
-            const double s = block_sum<256>(e[k], red);
-            if (threadIdx.x == 0) a.e_blk[blockIdx.x * FAST_E_STRIDE + k] = s;
+    if (a.drift_next) {
+        double4 lo, hi;
+        wave_aabb(pn.x, pn.y, pn.z, Rb, valid, lo, hi);
+        if (l == 0) {
+            a.bnd[2 * blockIdx.x] = lo;
+            a.bnd[2 * blockIdx.x + 1] = hi;
         }
     }
-    // profiling counters: one atomic per block each
-    __shared__ unsigned cc, cn;
-    if (threadIdx.x == 0) {
-        cc = 0;
-        cn = 0;
+    if (WRITE) {
+        // a 64-ball block lies inside one system, so its sums are per system
+        for (int k = 1; k < FAST_E_STRIDE; k++) {
+            const double s = wave_sum(e[k]);
+            if (l == 0) a.e_blk[blockIdx.x * FAST_E_STRIDE + k] = s;
+        }
+        if (l == 0) a.e_blk[blockIdx.x * FAST_E_STRIDE] = 0;
     }
-    __syncthreads();
-    if (g == 0 && ncontact) atomicAdd(&cc, ncontact);
-    if (nnear) atomicAdd(&cn, nnear);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (cc) atomicAdd(&a.counters[0], (unsigned long long)cc);
-        if (cn) atomicAdd(&a.counters[1], (unsigned long long)cn);
+}
+
+// Bounding boxes of the 64-ball blocks (one wavefront per block), for positions that
+// did not come from k_ball's drift.
+__global__ __launch_bounds__(TB) void k_bounds(State s, const int* blk_sys, const int* sys_off, const int* sys_n,
+                                               double4* bnd)
+{
+    const int l = threadIdx.x;
+    const int b = blockIdx.x * TB + l;
+    const int sy = blk_sys[blockIdx.x];
+    const bool valid = b - sys_off[sy] < sys_n[sy];
+    const int bc = valid ? b : sys_off[sy];
+    double4 lo, hi;
+    wave_aabb(s.f[PX][bc], s.f[PY][bc], s.f[PZ][bc], s.f[RAD][bc], valid, lo, hi);
+    if (l == 0) {
+        bnd[2 * blockIdx.x] = lo;
+        bnd[2 * blockIdx.x + 1] = hi;
     }
 }
 

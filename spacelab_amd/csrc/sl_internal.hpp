@@ -26,28 +26,13 @@ hipError_t launch_zero_i32(int* p, int n, hipStream_t st);
 
 // ---- fast path (Newton-3 tiles, fp64 or mixed): sl_kernels_fast.hip
 constexpr int FAST_TILE = 64;       // tile edge (balls) of the Newton-3 pair kernel
-constexpr int FAST_TILES_PER_BLOCK = 4;
-constexpr int FAST_MAX_NEAR = 32;   // near partners (touching or within h_min) per ball
 
 // Per-ball static data derived from R and m for the tile kernel.
 struct FastAux {
     double* q = nullptr;     // sqrt(Ha/6*64) R^3       (binary64 path: C_ij = q_i q_j)
     double* g = nullptr;     // sqrt(G) m               (G m_i m_j = g_i g_j)
-    float* r32 = nullptr;    // R / L0                  (mixed path, scaled units)
-    float* q32 = nullptr;    // (R / L0)^3
-    float* g32 = nullptr;    // m / M0
-};
-
-// One evaluated near pair, filed with each of its two balls: the partner, the
-// force and torque on this ball, and (for the reference orientation's A ball only)
-// the pair energy.  contact: 0 none, 1 touching and this ball is A, 2 touching and
-// this ball is B.
-struct NearEntry {
-    int j;
-    int contact;
-    double fx, fy, fz;
-    double tx, ty, tz;
-    double pe;
+    float* q32 = nullptr;    // sqrt(Kv/Kf) (R/L0)^3    (mixed path, scaled units)
+    float* g32 = nullptr;    // sqrt(Kg/Kf) m/M0
 };
 
 // Layout of one or more independent systems in one set of state arrays: system s
@@ -77,12 +62,21 @@ struct FastBuffers {
     std::vector<Phys> P;          // per system
     std::vector<int> blk_sys_h;   // per 64-ball block: system (host copy)
     std::vector<int> tile_sys_h;  // per tile: system (host copy)
-    double* alt_base = nullptr;   // second state set (ping-pong)
-    State alt{};
-    bool cur_is_alt = false;      // which set holds the current state
+    // The engine keeps its own two state sets (ping-pong), with each system's balls
+    // in Morton order of their positions at configuration, so that the 64-ball blocks
+    // are compact in space: most tiles then have no near pair.  perm maps engine slot
+    // -> canonical slot (the caller's order), within each system.
+    double* eng_base = nullptr;   // 2 * NFIELD * n_pad doubles
+    State eng[2] = {};
+    int cur = 0;                  // engine set holding the current state
+    int* perm = nullptr;          // device, n_pad
+    bool canon_valid = true;      // the canonical set holds the current state
+    bool eng_valid = false;       // the engine sets hold the current state
     bool drifted = false;         // the next step's drift was already applied
     FastAux aux{};
+    double4* bnd = nullptr;       // per 64-ball block: bounding box of the positions (lo, hi)
     double L0 = 1, M0 = 1;        // length and mass units of the mixed path
+    double Kf = 1;                // mixed path: force scale of the binary32 factor
     int* blk_sys = nullptr;       // device copies of the layout
     int* sys_off = nullptr;
     int* sys_n = nullptr;
@@ -91,13 +85,15 @@ struct FastBuffers {
     int4* tiles = nullptr;        // {I, J, system, first block of system}, I >= J
     int n_tiles = 0, cap_tiles = 0;
     double* part = nullptr;       // far-field force partials [K][xyz][n_pad]
-    int part_slots = 0;           // K capacity of part
-    int* nb_count = nullptr;      // near entries filed this step, per ball
-    NearEntry* nb_ent = nullptr;  // [n_pad][FAST_MAX_NEAR]
-    int* flags = nullptr;         // [0] near-list overflow, [1] non-finite state
-    unsigned long long* counters = nullptr;   // [0] contacts, [1] near entries (profiling)
-    double* e_blk = nullptr;      // per 32-ball block: PE(near), KE, p, L, contacts, nonfinite
-    double* pe_tile = nullptr;    // per tile: far-field PE (write steps)
+    unsigned long long* nbm = nullptr;   // near-partner masks [K][n_pad]
+    double* nd = nullptr;         // near-pair sums [K][F xyz, T xyz][n_pad] (where nbm != 0)
+    int part_slots = 0;           // K capacity of part, nbm and nd
+    int* flags = nullptr;         // [1] non-finite state
+    int* tile_nc = nullptr;       // per tile: touching pairs (write steps)
+    int* h_tile_nc = nullptr;     // pinned host copy
+    unsigned long long* tile_ctr = nullptr;   // per tile: touching and near pairs (running sums)
+    double* e_blk = nullptr;      // per 64-ball block: (unused), KE, p, L, (unused), nonfinite
+    double* pe_tile = nullptr;    // per tile: pair energy (write steps)
     double* h_e_blk = nullptr;    // pinned host copies
     double* h_pe_tile = nullptr;
     int precision = 0;
@@ -125,15 +121,17 @@ int fast_configure(FastBuffers& fb, State& canon, const FastLayout& lay, const P
                    hipStream_t st, std::string* err);
 // New physics parameters for the same layout (re-derives per-ball constants).
 int fast_set_phys(FastBuffers& fb, State& canon, const Phys* P, int precision, hipStream_t st, std::string* err);
-// Bring the canonical state set up to date (the fast path may hold it in the other set).
+// Bring the canonical state set up to date from the engine's sets.
 int fast_sync_to(FastBuffers& fb, State& canon, hipStream_t st, std::string* err);
+// The canonical set was changed by other code (the ordered path): the engine reloads it
+// before its next step.
+void fast_canon_changed(FastBuffers& fb);
 // Step-0 pass: forces of the current positions and the energies (one per system).
 int fast_init(FastBuffers& fb, State& canon, sl_energy* e_sys, hipStream_t st, std::string* err);
 // n_steps steps; the energies of write steps go to e_writes[w * n_sys + s] for the first
 // cap writes.  Returns the number of write steps or a negative error.
 int fast_steps(FastBuffers& fb, State& canon, int n_steps, long long* step_count, int write_every,
                sl_energy* e_writes, int cap, hipStream_t st, std::string* err);
-int fast_check(FastBuffers& fb, hipStream_t st, std::string* err);
 int fast_max_tile_blocks(int n_pad);
 void fast_clear_graphs(FastBuffers& fb);
 hipError_t fast_reset_counters(FastBuffers& fb, hipStream_t st);

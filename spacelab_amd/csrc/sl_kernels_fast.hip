@@ -3,17 +3,23 @@
 // system or a batch of independent systems laid out in one set of state arrays; the
 // kernels are in sl_fast_kernels.hpp.
 //
-// Per step: k_tiles (the pair triangles of every system in 64x64 tiles) then k_ball
-// (per-ball sums, near pairs, kick, energies and the next step's drift), ping-ponging
-// between two state sets.  Runs of steps between write steps are captured once as
-// HIP graphs and replayed.
+// The engine works on its own two state sets (ping-pong) in which each system's balls
+// are in Morton order of their positions at configuration; the caller's (canonical)
+// set is refreshed on demand.  Per step: k_tiles (the far-field sweep of every tile,
+// with the per-pair near test only where the blocks' bounding boxes come within h_min;
+// near pairs are recorded as partner masks), k_near (the near pairs of the tiles with
+// any, in the reference arithmetic), and k_ball (per-ball sums, kick, energies, the
+// next step's drift and block bounds).  Runs of steps between write steps are captured
+// once as HIP graphs and replayed.
 #include "spacelab/sl_api.h"
 #include "sl_fast_kernels.hpp"
 #include "sl_internal.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
+#include <numeric>
 #include <vector>
 
 namespace sl {
@@ -23,7 +29,7 @@ using namespace fastk;
 namespace {
 
 // Padding slots (outside every system's balls): far apart, zero radius and zero
-// cohesion and gravity weights, so the tile kernel can read whole 64-ball blocks
+// cohesion and gravity weights, so the tile kernels can read whole 64-ball blocks
 // without tests.  With R = 0 a padding pair's force and energy terms are exactly
 // zero (s = t, so log(s/t) = 0), which keeps a system's sums independent of where
 // its padding lies.
@@ -44,12 +50,12 @@ __global__ void k_pad_sys(State s, FastAux aux, const int* blk_sys, const int* s
     s.f[MOI][b] = 1.0;
     aux.q[b] = 0;
     aux.g[b] = 0;
-    aux.r32[b] = 0.f;
     aux.q32[b] = 0.f;
     aux.g32[b] = 0.f;
 }
 
-__global__ void k_aux(State s, FastAux aux, int n, double sqrtC, double sqrtG, double L0inv, double M0inv)
+__global__ void k_aux(State s, FastAux aux, int n, double sqrtC, double sqrtG, double L0inv, double M0inv,
+                      double wq, double wg)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n) return;
@@ -57,16 +63,26 @@ __global__ void k_aux(State s, FastAux aux, int n, double sqrtC, double sqrtG, d
     aux.q[b] = sqrtC * R * R * R;
     aux.g[b] = sqrtG * m;
     const double r = R * L0inv;
-    aux.r32[b] = (float)r;
-    aux.q32[b] = (float)(r * r * r);
-    aux.g32[b] = (float)(m * M0inv);
+    aux.q32[b] = (float)(wq * (r * r * r));
+    aux.g32[b] = (float)(wg * (m * M0inv));
 }
 
-__global__ void k_copy_fields(State dst, State src, int n, int f0, int f1)
+// dst[k] = src[perm[k]] (to engine order) for fields [f0, f1)
+__global__ void k_gather(State dst, State src, const int* perm, int n, int f0, int f1)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n) return;
-    for (int f = f0; f < f1; f++) dst.f[f][b] = src.f[f][b];
+    const int c = perm[b];
+    for (int f = f0; f < f1; f++) dst.f[f][b] = src.f[f][c];
+}
+
+// dst[perm[k]] = src[k] (to canonical order) for fields [f0, f1)
+__global__ void k_scatter(State dst, State src, const int* perm, int n, int f0, int f1)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const int c = perm[b];
+    for (int f = f0; f < f1; f++) dst.f[f][c] = src.f[f][b];
 }
 
 // Lane-layout probe for the DPP rotation the tile kernels rely on.
@@ -78,10 +94,7 @@ __global__ void k_probe_rot(int* out)
     out[128 + l] = (int)rotk((double)l, 31);
 }
 
-constexpr int GRAPH_MAX_STEPS = 128;   // steps per captured graph (two kernels each)
-
-State& cur_state(FastBuffers& fb, State& canon) { return fb.cur_is_alt ? fb.alt : canon; }
-State& nxt_state(FastBuffers& fb, State& canon) { return fb.cur_is_alt ? canon : fb.alt; }
+constexpr int GRAPH_MAX_STEPS = 128;   // steps per captured graph
 
 int err_hip(std::string* err, const char* what)
 {
@@ -89,58 +102,73 @@ int err_hip(std::string* err, const char* what)
     return SL_E_HIP;
 }
 
-template <bool WRITE, bool GRAV, int SPLIT>
-void launch_tiles(const TileArgs& a, int precision, hipStream_t st)
+template <int PREC, bool WRITE, bool GRAV, int SPLIT>
+void launch_far(const TileArgs& a, hipStream_t st)
 {
     constexpr int TPB = NW / SPLIT;
-    const int blocks = (a.n_tiles + TPB - 1) / TPB;
-    if (precision == SL_PREC_MIXED)
-        k_tiles_mixed<WRITE, GRAV, SPLIT><<<blocks, NT, 0, st>>>(a);
-    else
-        k_tiles_f64<WRITE, GRAV, SPLIT><<<blocks, NT, 0, st>>>(a);
+    k_tiles<PREC, WRITE, GRAV, SPLIT><<<(a.n_tiles + TPB - 1) / TPB, NT, 0, st>>>(a);
 }
 
 template <int SPLIT>
-void launch_tiles_split(const TileArgs& a, int precision, bool write, bool grav, hipStream_t st)
+void launch_far_split(const TileArgs& a, int precision, bool write, bool grav, hipStream_t st)
 {
-    if (write)
-        grav ? launch_tiles<true, true, SPLIT>(a, precision, st) : launch_tiles<true, false, SPLIT>(a, precision, st);
-    else
-        grav ? launch_tiles<false, true, SPLIT>(a, precision, st) : launch_tiles<false, false, SPLIT>(a, precision, st);
+    if (precision == SL_PREC_MIXED) {
+        if (write)
+            grav ? launch_far<SL_PREC_MIXED, true, true, SPLIT>(a, st) : launch_far<SL_PREC_MIXED, true, false, SPLIT>(a, st);
+        else
+            grav ? launch_far<SL_PREC_MIXED, false, true, SPLIT>(a, st) : launch_far<SL_PREC_MIXED, false, false, SPLIT>(a, st);
+    } else {
+        if (write)
+            grav ? launch_far<SL_PREC_FP64, true, true, SPLIT>(a, st) : launch_far<SL_PREC_FP64, true, false, SPLIT>(a, st);
+        else
+            grav ? launch_far<SL_PREC_FP64, false, true, SPLIT>(a, st) : launch_far<SL_PREC_FP64, false, false, SPLIT>(a, st);
+    }
 }
 
-void launch_tiles_any(const TileArgs& a, int split, int precision, bool write, bool grav, hipStream_t st)
+void launch_far_any(const TileArgs& a, int split, int precision, bool write, bool grav, hipStream_t st)
 {
     if (split == 4)
-        launch_tiles_split<4>(a, precision, write, grav, st);
+        launch_far_split<4>(a, precision, write, grav, st);
     else if (split == 2)
-        launch_tiles_split<2>(a, precision, write, grav, st);
+        launch_far_split<2>(a, precision, write, grav, st);
     else
-        launch_tiles_split<1>(a, precision, write, grav, st);
+        launch_far_split<1>(a, precision, write, grav, st);
 }
 
-TileArgs tile_args(FastBuffers& fb, State& c)
+void launch_near(const TileArgs& a, bool write, hipStream_t st)
 {
+    const int blocks = a.n_tiles;   // one workgroup per tile
+    if (write)
+        k_near<true><<<blocks, NT, 0, st>>>(a);
+    else
+        k_near<false><<<blocks, NT, 0, st>>>(a);
+}
+
+TileArgs tile_args(FastBuffers& fb)
+{
+    const State& c = fb.eng[fb.cur];
     TileArgs a;
     a.px = c.f[PX];
     a.py = c.f[PY];
     a.pz = c.f[PZ];
     a.R = c.f[RAD];
     a.aux = fb.aux;
-    a.cur = c;
     a.sysP = fb.sysP;
+    a.bnd = fb.bnd;
     a.n_pad = fb.n_pad;
     a.n_tiles = fb.n_tiles;
     a.tiles = fb.tiles;
-    const Phys& P = fb.P[0];   // G, Ha and h_min are common to a batch
-    a.L0inv2 = 1.0 / (fb.L0 * fb.L0);
-    a.Kv = (P.Ha / 6 * 64) / (fb.L0 * fb.L0);
-    a.Kg = P.G * fb.M0 * fb.M0;
+    a.L0inv = (float)(1.0 / fb.L0);
+    a.L0inv2 = (float)(1.0 / (fb.L0 * fb.L0));
+    a.Kf = fb.Kf;
     a.part = fb.part;
-    a.nb_count = fb.nb_count;
-    a.nb_ent = fb.nb_ent;
-    a.flags = fb.flags;
+    a.nbm = fb.nbm;
+    a.nd = fb.nd;
     a.pe_tile = fb.pe_tile;
+    a.tile_nc = fb.tile_nc;
+    a.tile_ctr = fb.tile_ctr;
+    a.cur = c;
+    a.perm = fb.perm;
     return a;
 }
 
@@ -151,11 +179,11 @@ int split_key(const FastBuffers& fb) { return fb.precision * 2 + (gravity_on(fb)
 // The tile split has been chosen for the current precision and gravity setting.
 bool split_ready(const FastBuffers& fb) { return fb.split != 0 && fb.split_key == split_key(fb); }
 
-// Autotuner: the number of wavefronts sharing one 64x64 tile trades per-wave set-up
+// Autotuner: the number of wavefronts sharing one far tile trades per-wave set-up
 // against occupancy and load balance, which depend on the systems' sizes and the
 // machine.  Each choice is warmed up once and then timed over five launches on the
-// live state (HIP events on the engine's stream); the fastest is kept.  The near
-// entries these trial launches file are cleared again.
+// live state (HIP events on the engine's stream); the fastest is kept.  The trial
+// launches only write partial sums that the real launch overwrites.
 int tune_split(FastBuffers& fb, const TileArgs& a, hipStream_t st)
 {
     if (fb.forced_split == 1 || fb.forced_split == 2 || fb.forced_split == 4) return fb.forced_split;
@@ -170,13 +198,9 @@ int tune_split(FastBuffers& fb, const TileArgs& a, hipStream_t st)
     float best_ms = 1e30f;
     const int choices[3] = {1, 2, 4};
     for (int c : choices) {
-        launch_tiles_any(a, c, fb.precision, false, grav, st);
-        (void)launch_zero_i32(fb.nb_count, fb.n_pad, st);
+        launch_far_any(a, c, fb.precision, false, grav, st);
         (void)hipEventRecord(e0, st);
-        for (int rep = 0; rep < 5; rep++) {
-            launch_tiles_any(a, c, fb.precision, false, grav, st);
-            (void)launch_zero_i32(fb.nb_count, fb.n_pad, st);
-        }
+        for (int rep = 0; rep < 5; rep++) launch_far_any(a, c, fb.precision, false, grav, st);
         (void)hipEventRecord(e1, st);
         if (hipEventSynchronize(e1) != hipSuccess) break;
         float ms = 0;
@@ -186,35 +210,33 @@ int tune_split(FastBuffers& fb, const TileArgs& a, hipStream_t st)
             best = c;
         }
     }
-    (void)launch_zero_i32(fb.flags, 4, st);
     (void)hipStreamSynchronize(st);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return best;
 }
 
-int launch_tiles_step(FastBuffers& fb, State& canon, bool write, hipStream_t st, std::string* err)
+int launch_tiles_step(FastBuffers& fb, bool write, hipStream_t st, std::string* err)
 {
-    State& c = cur_state(fb, canon);
-    const TileArgs a = tile_args(fb, c);
+    const TileArgs a = tile_args(fb);
     if (!split_ready(fb)) {
         fb.split = tune_split(fb, a, st);
         fb.split_key = split_key(fb);
         fast_clear_graphs(fb);
     }
     if (fb.hook.before) fb.hook.before(fb.hook.user, 0);
-    launch_tiles_any(a, fb.split, fb.precision, write, gravity_on(fb), st);
+    launch_far_any(a, fb.split, fb.precision, write, gravity_on(fb), st);
+    launch_near(a, write, st);
     if (fb.hook.after) fb.hook.after(fb.hook.user, 0);
     if (hipGetLastError() != hipSuccess) return err_hip(err, "tile kernel launch");
     return 0;
 }
 
-int launch_ball_step(FastBuffers& fb, State& canon, bool write, bool kick, bool drift_next, hipStream_t st,
-                     std::string* err)
+int launch_ball_step(FastBuffers& fb, bool write, bool kick, bool drift_next, hipStream_t st, std::string* err)
 {
     BallArgs a;
-    a.cur = cur_state(fb, canon);
-    a.nxt = nxt_state(fb, canon);
+    a.cur = fb.eng[fb.cur];
+    a.nxt = fb.eng[1 - fb.cur];
     a.sysP = fb.sysP;
     a.blk_sys = fb.blk_sys;
     a.sys_off = fb.sys_off;
@@ -223,39 +245,48 @@ int launch_ball_step(FastBuffers& fb, State& canon, bool write, bool kick, bool 
     a.n_used = fb.lay.n_used;
     a.n_pad = fb.n_pad;
     a.part = fb.part;
-    a.nb_count = fb.nb_count;
-    a.nb_ent = fb.nb_ent;
-    a.write = write;
+    a.nbm = fb.nbm;
+    a.nd = fb.nd;
     a.kick = kick;
     a.drift_next = drift_next;
-    a.compensated = fb.precision == SL_PREC_MIXED;
+    a.bnd = fb.bnd;
     a.e_blk = fb.e_blk;
     a.flags = fb.flags;
-    a.counters = fb.counters;
-    const int blocks = (fb.lay.n_used + BPB - 1) / BPB;
+    const int blocks = fb.lay.n_used / TB;
     if (fb.hook.before) fb.hook.before(fb.hook.user, 1);
-    k_ball<<<blocks, 256, 0, st>>>(a);
+    if (fb.precision == SL_PREC_MIXED)
+        write ? k_ball<true, true><<<blocks, KB_T, 0, st>>>(a) : k_ball<true, false><<<blocks, KB_T, 0, st>>>(a);
+    else
+        write ? k_ball<false, true><<<blocks, KB_T, 0, st>>>(a) : k_ball<false, false><<<blocks, KB_T, 0, st>>>(a);
     if (fb.hook.after) fb.hook.after(fb.hook.user, 1);
     if (hipGetLastError() != hipSuccess) return err_hip(err, "ball kernel launch");
-    fb.cur_is_alt = !fb.cur_is_alt;
+    fb.cur = 1 - fb.cur;
     fb.drifted = drift_next;
     return 0;
 }
 
-int launch_drift_sys(FastBuffers& fb, State& canon, hipStream_t st, std::string* err)
+int launch_bounds(FastBuffers& fb, hipStream_t st, std::string* err)
 {
-    k_drift_sys<<<(fb.lay.n_used + 255) / 256, 256, 0, st>>>(cur_state(fb, canon), fb.blk_sys, fb.sys_off,
-                                                             fb.sys_n, fb.sysP, fb.lay.n_used);
-    if (hipGetLastError() != hipSuccess) return err_hip(err, "drift");
+    k_bounds<<<fb.lay.n_used / TB, TB, 0, st>>>(fb.eng[fb.cur], fb.blk_sys, fb.sys_off, fb.sys_n, fb.bnd);
+    if (hipGetLastError() != hipSuccess) return err_hip(err, "block bounds");
     return 0;
 }
 
+int launch_drift_sys(FastBuffers& fb, hipStream_t st, std::string* err)
+{
+    k_drift_sys<<<(fb.lay.n_used + 255) / 256, 256, 0, st>>>(fb.eng[fb.cur], fb.blk_sys, fb.sys_off, fb.sys_n,
+                                                             fb.sysP, fb.lay.n_used);
+    if (hipGetLastError() != hipSuccess) return err_hip(err, "drift");
+    return launch_bounds(fb, st, err);
+}
+
 // Read back the partials of a write step and sum them per system, in a fixed order:
-// tiles in table order, then 32-ball blocks in ascending order.
+// tiles in table order, then 64-ball blocks in ascending order.
 int read_energies(FastBuffers& fb, sl_energy* out, hipStream_t st, std::string* err)
 {
-    const int nblk = (fb.lay.n_used + BPB - 1) / BPB;
+    const int nblk = fb.lay.n_used / BPB;
     if (hipMemcpyAsync(fb.h_pe_tile, fb.pe_tile, sizeof(double) * fb.n_tiles, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(fb.h_tile_nc, fb.tile_nc, sizeof(int) * fb.n_tiles, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipMemcpyAsync(fb.h_e_blk, fb.e_blk, sizeof(double) * FAST_E_STRIDE * nblk, hipMemcpyDeviceToHost, st) !=
             hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
@@ -263,12 +294,14 @@ int read_energies(FastBuffers& fb, sl_energy* out, hipStream_t st, std::string* 
     const int ns = fb.lay.n_sys;
     std::vector<sl_energy> e(ns);
     for (auto& x : e) x = sl_energy{};
-    for (int t = 0; t < fb.n_tiles; t++) e[fb.tile_sys_h[t]].PE += fb.h_pe_tile[t];
+    for (int t = 0; t < fb.n_tiles; t++) {
+        e[fb.tile_sys_h[t]].PE += fb.h_pe_tile[t];
+        e[fb.tile_sys_h[t]].contacts += fb.h_tile_nc[t];
+    }
     for (int b = 0; b < nblk; b++) {
         const int s = fb.blk_sys_h[(b * BPB) / TB];
         const double* v = fb.h_e_blk + (size_t)b * FAST_E_STRIDE;
         sl_energy& x = e[s];
-        x.PE += v[0];
         x.KE += v[1];
         x.p[0] += v[2];
         x.p[1] += v[3];
@@ -276,7 +309,6 @@ int read_energies(FastBuffers& fb, sl_energy* out, hipStream_t st, std::string* 
         x.L[0] += v[5];
         x.L[1] += v[6];
         x.L[2] += v[7];
-        x.contacts += (long long)v[8];
         if (v[9] > 0) x.nonfinite = 1;
     }
     if (out)
@@ -285,35 +317,35 @@ int read_energies(FastBuffers& fb, sl_energy* out, hipStream_t st, std::string* 
 }
 
 // One step launched directly.
-int direct_step(FastBuffers& fb, State& canon, bool write, bool drift_next, hipStream_t st, std::string* err)
+int direct_step(FastBuffers& fb, bool write, bool drift_next, hipStream_t st, std::string* err)
 {
     int rc;
-    if (!fb.drifted && (rc = launch_drift_sys(fb, canon, st, err))) return rc;
+    if (!fb.drifted && (rc = launch_drift_sys(fb, st, err))) return rc;
     fb.drifted = false;
-    if ((rc = launch_tiles_step(fb, canon, write, st, err))) return rc;
-    return launch_ball_step(fb, canon, write, true, drift_next, st, err);
+    if ((rc = launch_tiles_step(fb, write, st, err))) return rc;
+    return launch_ball_step(fb, write, true, drift_next, st, err);
 }
 
 // A run of `len` steps that starts drifted and ends drifted (the last may be a
 // write step), captured once per (length, write, state parity) and replayed.
-int graph_segment(FastBuffers& fb, State& canon, int len, bool write_last, hipStream_t st, std::string* err)
+int graph_segment(FastBuffers& fb, int len, bool write_last, hipStream_t st, std::string* err)
 {
-    const FastBuffers::GraphKey key{len, write_last ? 1 : 0, fb.cur_is_alt ? 1 : 0};
+    const FastBuffers::GraphKey key{len, write_last ? 1 : 0, fb.cur};
     auto it = fb.graphs.find(key);
     if (it == fb.graphs.end()) {
         if (fb.graphs.size() >= 16) fast_clear_graphs(fb);
-        const bool parity0 = fb.cur_is_alt;
+        const int cur0 = fb.cur;
         hipGraph_t g = nullptr;
         if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
             return err_hip(err, "begin graph capture");
         int rc = 0;
         for (int k = 0; k < len && !rc; k++) {
             const bool w = write_last && k == len - 1;
-            rc = launch_tiles_step(fb, canon, w, st, err);
-            if (!rc) rc = launch_ball_step(fb, canon, w, true, true, st, err);
+            rc = launch_tiles_step(fb, w, st, err);
+            if (!rc) rc = launch_ball_step(fb, w, true, true, st, err);
         }
         const hipError_t ec = hipStreamEndCapture(st, &g);
-        fb.cur_is_alt = parity0;   // capture recorded the launches; nothing ran yet
+        fb.cur = cur0;   // capture recorded the launches; nothing ran yet
         fb.drifted = true;
         if (rc) {
             if (g) (void)hipGraphDestroy(g);
@@ -327,9 +359,76 @@ int graph_segment(FastBuffers& fb, State& canon, int len, bool write_last, hipSt
         it = fb.graphs.emplace(key, ex).first;
     }
     if (hipGraphLaunch(it->second, st) != hipSuccess) return err_hip(err, "graph launch");
-    if (len & 1) fb.cur_is_alt = !fb.cur_is_alt;
+    if (len & 1) fb.cur = 1 - fb.cur;
     fb.drifted = true;
     return 0;
+}
+
+// Morton code of a point already scaled to [0, 2^21) per axis.
+uint64_t morton3(uint32_t x, uint32_t y, uint32_t z)
+{
+    auto spread = [](uint64_t v) {
+        v &= 0x1fffff;
+        v = (v | v << 32) & 0x1f00000000ffffULL;
+        v = (v | v << 16) & 0x1f0000ff0000ffULL;
+        v = (v | v << 8) & 0x100f00f00f00f00fULL;
+        v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+        v = (v | v << 2) & 0x1249249249249249ULL;
+        return v;
+    };
+    return spread(x) | spread(y) << 1 | spread(z) << 2;
+}
+
+// Engine slot -> canonical slot: each system's balls sorted by the Morton code of
+// their positions (ties by canonical index); padding slots map to themselves.
+std::vector<int> morton_perm(const FastLayout& L, const std::vector<double>& x, const std::vector<double>& y,
+                             const std::vector<double>& z, int n_pad)
+{
+    std::vector<int> perm(n_pad);
+    std::iota(perm.begin(), perm.end(), 0);
+    for (int s = 0; s < L.n_sys; s++) {
+        const int o = L.off[s], n = L.n[s];
+        double lo[3] = {x[o], y[o], z[o]}, hi[3] = {x[o], y[o], z[o]};
+        for (int k = 0; k < n; k++) {
+            const double p[3] = {x[o + k], y[o + k], z[o + k]};
+            for (int d = 0; d < 3; d++) {
+                lo[d] = std::min(lo[d], p[d]);
+                hi[d] = std::max(hi[d], p[d]);
+            }
+        }
+        double ext = 0;
+        for (int d = 0; d < 3; d++) ext = std::max(ext, hi[d] - lo[d]);
+        const double sc = ext > 0 ? 2097151.0 / ext : 0.0;
+        std::vector<std::pair<uint64_t, int>> key(n);
+        for (int k = 0; k < n; k++) {
+            const uint32_t q0 = (uint32_t)((x[o + k] - lo[0]) * sc);
+            const uint32_t q1 = (uint32_t)((y[o + k] - lo[1]) * sc);
+            const uint32_t q2 = (uint32_t)((z[o + k] - lo[2]) * sc);
+            key[k] = {morton3(q0, q1, q2), o + k};
+        }
+        std::sort(key.begin(), key.end());
+        for (int k = 0; k < n; k++) perm[o + k] = key[k].second;
+    }
+    return perm;
+}
+
+// Refresh the engine's current set from the canonical set (dynamic fields).
+int load_from_canon(FastBuffers& fb, State& canon, hipStream_t st, std::string* err)
+{
+    k_gather<<<(fb.lay.n_used + 255) / 256, 256, 0, st>>>(fb.eng[fb.cur], canon, fb.perm, fb.lay.n_used, PX, RAD);
+    if (hipGetLastError() != hipSuccess) return err_hip(err, "gather state");
+    k_pad_sys<<<(fb.n_pad + 255) / 256, 256, 0, st>>>(fb.eng[fb.cur], fb.aux, fb.blk_sys, fb.sys_off, fb.sys_n,
+                                                      fb.lay.n_used, fb.n_pad);
+    if (hipGetLastError() != hipSuccess) return err_hip(err, "padding");
+    fb.eng_valid = true;
+    fb.drifted = false;
+    return 0;
+}
+
+int ensure_engine(FastBuffers& fb, State& canon, hipStream_t st, std::string* err)
+{
+    if (fb.eng_valid) return 0;
+    return load_from_canon(fb, canon, st, err);
 }
 
 }  // namespace
@@ -363,29 +462,28 @@ int fast_alloc(FastBuffers& fb, int n_pad, int max_sys)
     fb.n_pad = n_pad;
     fb.max_sys = max_sys;
     const int nblocks = (n_pad + TB - 1) / TB;
-    if (hipMalloc(&fb.alt_base, sizeof(double) * (size_t)NFIELD * n_pad) != hipSuccess) return -1;
+    if (hipMalloc(&fb.eng_base, sizeof(double) * 2 * (size_t)NFIELD * n_pad) != hipSuccess) return -1;
+    if (hipMalloc(&fb.perm, sizeof(int) * n_pad) != hipSuccess) return -1;
     if (hipMalloc(&fb.aux.q, sizeof(double) * n_pad) != hipSuccess) return -1;
     if (hipMalloc(&fb.aux.g, sizeof(double) * n_pad) != hipSuccess) return -1;
-    if (hipMalloc(&fb.aux.r32, sizeof(float) * n_pad) != hipSuccess) return -1;
     if (hipMalloc(&fb.aux.q32, sizeof(float) * n_pad) != hipSuccess) return -1;
     if (hipMalloc(&fb.aux.g32, sizeof(float) * n_pad) != hipSuccess) return -1;
+    if (hipMalloc(&fb.bnd, sizeof(double4) * 2 * nblocks) != hipSuccess) return -1;
     if (hipMalloc(&fb.blk_sys, sizeof(int) * nblocks) != hipSuccess) return -1;
     if (hipMalloc(&fb.sys_off, sizeof(int) * max_sys) != hipSuccess) return -1;
     if (hipMalloc(&fb.sys_n, sizeof(int) * max_sys) != hipSuccess) return -1;
     if (hipMalloc(&fb.sys_nb, sizeof(int) * max_sys) != hipSuccess) return -1;
     if (hipMalloc(&fb.sysP, sizeof(Phys) * max_sys) != hipSuccess) return -1;
-    if (hipMalloc(&fb.nb_count, sizeof(int) * n_pad) != hipSuccess) return -1;
-    if (hipMalloc(&fb.nb_ent, sizeof(NearEntry) * (size_t)n_pad * MAXNB) != hipSuccess) return -1;
     if (hipMalloc(&fb.flags, sizeof(int) * 4) != hipSuccess) return -1;
-    if (hipMalloc(&fb.counters, sizeof(unsigned long long) * 2) != hipSuccess) return -1;
     const int eblocks = (n_pad + BPB - 1) / BPB;
     if (hipMalloc(&fb.e_blk, sizeof(double) * FAST_E_STRIDE * eblocks) != hipSuccess) return -1;
     if (hipHostMalloc(&fb.h_e_blk, sizeof(double) * FAST_E_STRIDE * eblocks, 0) != hipSuccess) return -1;
-    if (hipMemset(fb.alt_base, 0, sizeof(double) * (size_t)NFIELD * n_pad) != hipSuccess) return -1;
-    if (hipMemset(fb.nb_count, 0, sizeof(int) * n_pad) != hipSuccess) return -1;
+    if (hipMemset(fb.eng_base, 0, sizeof(double) * 2 * (size_t)NFIELD * n_pad) != hipSuccess) return -1;
     if (hipMemset(fb.flags, 0, sizeof(int) * 4) != hipSuccess) return -1;
-    if (hipMemset(fb.counters, 0, sizeof(unsigned long long) * 2) != hipSuccess) return -1;
-    for (int f = 0; f < NFIELD; f++) fb.alt.f[f] = fb.alt_base + (size_t)f * n_pad;
+    for (int k = 0; k < 2; k++)
+        for (int f = 0; f < NFIELD; f++) fb.eng[k].f[f] = fb.eng_base + ((size_t)k * NFIELD + f) * n_pad;
+    // radius, mass and moment of inertia are shared by both engine sets
+    for (int f = RAD; f < NFIELD; f++) fb.eng[1].f[f] = fb.eng[0].f[f];
     return 0;
 }
 
@@ -398,13 +496,14 @@ void fast_clear_graphs(FastBuffers& fb)
 void fast_free(FastBuffers& fb)
 {
     fast_clear_graphs(fb);
-    void* ptrs[] = {fb.alt_base, fb.aux.q, fb.aux.g, fb.aux.r32, fb.aux.q32, fb.aux.g32, fb.blk_sys,
-                    fb.sys_off, fb.sys_n, fb.sys_nb, fb.sysP, fb.tiles, fb.part, fb.nb_count, fb.nb_ent,
-                    fb.flags, fb.counters, fb.e_blk, fb.pe_tile};
+    void* ptrs[] = {fb.eng_base, fb.perm, fb.aux.q, fb.aux.g, fb.aux.q32, fb.aux.g32, fb.bnd, fb.blk_sys,
+                    fb.sys_off, fb.sys_n, fb.sys_nb, fb.sysP, fb.tiles, fb.part, fb.nbm, fb.nd, fb.flags,
+                    fb.tile_nc, fb.tile_ctr, fb.e_blk, fb.pe_tile};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (fb.h_e_blk) (void)hipHostFree(fb.h_e_blk);
     if (fb.h_pe_tile) (void)hipHostFree(fb.h_pe_tile);
+    if (fb.h_tile_nc) (void)hipHostFree(fb.h_tile_nc);
     fb = FastBuffers{};
 }
 
@@ -432,13 +531,16 @@ int fast_set_phys(FastBuffers& fb, State& canon, const Phys* P, int precision, h
     std::nth_element(mv.begin(), mv.begin() + h, mv.end());
     fb.L0 = Rv[h];
     fb.M0 = mv[h];
+    // mixed path: force = Kf (q_i q_j / (s t)^2 + g_i g_j / d^3) (r / L0) in units of L0, M0
+    const double Kv = (P[0].Ha / 6 * 64) / fb.L0, Kg = P[0].G * fb.M0 * fb.M0 / (fb.L0 * fb.L0);
+    fb.Kf = Kv > 0 ? Kv : (Kg > 0 ? Kg : 1.0);
     const double C = P[0].Ha / 6 * 64;
-    k_aux<<<(L.n_used + 255) / 256, 256, 0, st>>>(canon, fb.aux, L.n_used, std::sqrt(C), std::sqrt(P[0].G),
-                                                  1.0 / fb.L0, 1.0 / fb.M0);
-    k_pad_sys<<<(fb.n_pad + 255) / 256, 256, 0, st>>>(canon, fb.aux, fb.blk_sys, fb.sys_off, fb.sys_n, L.n_used,
-                                                      fb.n_pad);
-    k_pad_sys<<<(fb.n_pad + 255) / 256, 256, 0, st>>>(fb.alt, fb.aux, fb.blk_sys, fb.sys_off, fb.sys_n, L.n_used,
-                                                      fb.n_pad);
+    k_aux<<<(L.n_used + 255) / 256, 256, 0, st>>>(fb.eng[0], fb.aux, L.n_used, std::sqrt(C), std::sqrt(P[0].G),
+                                                  1.0 / fb.L0, 1.0 / fb.M0, std::sqrt(Kv / fb.Kf),
+                                                  std::sqrt(Kg / fb.Kf));
+    for (int k = 0; k < 2; k++)
+        k_pad_sys<<<(fb.n_pad + 255) / 256, 256, 0, st>>>(fb.eng[k], fb.aux, fb.blk_sys, fb.sys_off, fb.sys_n,
+                                                          L.n_used, fb.n_pad);
     if (hipGetLastError() != hipSuccess) return err_hip(err, "per-ball constants");
     fast_clear_graphs(fb);
     return 0;
@@ -453,11 +555,7 @@ int fast_configure(FastBuffers& fb, State& canon, const FastLayout& lay, const P
     }
     const bool same = fb.lay.n_sys == lay.n_sys && fb.lay.n == lay.n;
     fb.lay = lay;
-    // the radius/mass/moi arrays are shared by both state sets
-    fb.alt.f[RAD] = canon.f[RAD];
-    fb.alt.f[MASS] = canon.f[MASS];
-    fb.alt.f[MOI] = canon.f[MOI];
-    fb.cur_is_alt = false;
+    fb.cur = 0;
     fb.drifted = false;
     if (!same) {
         const int nblocks = lay.n_used / TB;
@@ -477,22 +575,37 @@ int fast_configure(FastBuffers& fb, State& canon, const FastLayout& lay, const P
         if (nt > fb.cap_tiles) {
             if (fb.tiles) (void)hipFree(fb.tiles);
             if (fb.pe_tile) (void)hipFree(fb.pe_tile);
+            if (fb.tile_nc) (void)hipFree(fb.tile_nc);
+            if (fb.tile_ctr) (void)hipFree(fb.tile_ctr);
             if (fb.h_pe_tile) (void)hipHostFree(fb.h_pe_tile);
+            if (fb.h_tile_nc) (void)hipHostFree(fb.h_tile_nc);
             fb.tiles = nullptr;
             fb.pe_tile = nullptr;
+            fb.tile_nc = nullptr;
+            fb.tile_ctr = nullptr;
             fb.h_pe_tile = nullptr;
+            fb.h_tile_nc = nullptr;
             fb.cap_tiles = 0;
             if (hipMalloc(&fb.tiles, sizeof(int4) * nt) != hipSuccess ||
                 hipMalloc(&fb.pe_tile, sizeof(double) * nt) != hipSuccess ||
-                hipHostMalloc(&fb.h_pe_tile, sizeof(double) * nt, 0) != hipSuccess)
+                hipMalloc(&fb.tile_nc, sizeof(int) * nt) != hipSuccess ||
+                hipMalloc(&fb.tile_ctr, sizeof(unsigned long long) * 2 * nt) != hipSuccess ||
+                hipHostMalloc(&fb.h_pe_tile, sizeof(double) * nt, 0) != hipSuccess ||
+                hipHostMalloc(&fb.h_tile_nc, sizeof(int) * nt, 0) != hipSuccess)
                 return err_hip(err, "tile table allocation");
             fb.cap_tiles = nt;
         }
         if (lay.nb_max > fb.part_slots) {
             if (fb.part) (void)hipFree(fb.part);
+            if (fb.nbm) (void)hipFree(fb.nbm);
+            if (fb.nd) (void)hipFree(fb.nd);
             fb.part = nullptr;
+            fb.nbm = nullptr;
+            fb.nd = nullptr;
             fb.part_slots = 0;
-            if (hipMalloc(&fb.part, sizeof(double) * (size_t)lay.nb_max * 3 * fb.n_pad) != hipSuccess)
+            if (hipMalloc(&fb.part, sizeof(double) * (size_t)lay.nb_max * 3 * fb.n_pad) != hipSuccess ||
+                hipMalloc(&fb.nbm, sizeof(unsigned long long) * (size_t)lay.nb_max * fb.n_pad) != hipSuccess ||
+                hipMalloc(&fb.nd, sizeof(double) * (size_t)lay.nb_max * 6 * fb.n_pad) != hipSuccess)
                 return err_hip(err, "partial buffer allocation");
             fb.part_slots = lay.nb_max;
         }
@@ -507,43 +620,75 @@ int fast_configure(FastBuffers& fb, State& canon, const FastLayout& lay, const P
         fb.n_tiles = nt;
         fb.split = 0;   // re-tune for the new sizes
     }
-    if (launch_zero_i32(fb.nb_count, fb.n_pad, st) != hipSuccess || launch_zero_i32(fb.flags, 4, st) != hipSuccess)
-        return err_hip(err, "clear near lists");
-    return fast_set_phys(fb, canon, P, precision, st, err);
+    if (launch_zero_u64(fb.tile_ctr, 2 * fb.cap_tiles, st) != hipSuccess ||
+        launch_zero_i32(fb.flags, 4, st) != hipSuccess)
+        return err_hip(err, "clear counters");
+    // Morton order of the current positions, per system
+    const int nu = lay.n_used;
+    std::vector<double> x(nu), y(nu), z(nu);
+    if (hipMemcpyAsync(x.data(), canon.f[PX], sizeof(double) * nu, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(y.data(), canon.f[PY], sizeof(double) * nu, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(z.data(), canon.f[PZ], sizeof(double) * nu, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return err_hip(err, "read positions");
+    const std::vector<int> perm = morton_perm(lay, x, y, z, fb.n_pad);
+    if (hipMemcpyAsync(fb.perm, perm.data(), sizeof(int) * fb.n_pad, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return err_hip(err, "permutation");
+    k_gather<<<(nu + 255) / 256, 256, 0, st>>>(fb.eng[0], canon, fb.perm, nu, RAD, NFIELD);
+    if (hipGetLastError() != hipSuccess) return err_hip(err, "gather constants");
+    int rc = fast_set_phys(fb, canon, P, precision, st, err);
+    if (rc) return rc;
+    rc = load_from_canon(fb, canon, st, err);
+    if (rc) return rc;
+    fb.canon_valid = true;
+    return 0;
 }
 
 int fast_sync_to(FastBuffers& fb, State& canon, hipStream_t st, std::string* err)
 {
-    if (!fb.cur_is_alt) return 0;
-    k_copy_fields<<<(fb.lay.n_used + 255) / 256, 256, 0, st>>>(canon, fb.alt, fb.lay.n_used, PX, RAD);
-    if (hipGetLastError() != hipSuccess) return err_hip(err, "copy state");
-    fb.cur_is_alt = false;
+    if (fb.canon_valid) return 0;
+    k_scatter<<<(fb.lay.n_used + 255) / 256, 256, 0, st>>>(canon, fb.eng[fb.cur], fb.perm, fb.lay.n_used, PX, RAD);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return err_hip(err, "copy state");
+    fb.canon_valid = true;
     return 0;
+}
+
+void fast_canon_changed(FastBuffers& fb)
+{
+    fb.eng_valid = false;
+    fb.canon_valid = true;
 }
 
 int fast_init(FastBuffers& fb, State& canon, sl_energy* e_sys, hipStream_t st, std::string* err)
 {
-    if (launch_copy_prev(cur_state(fb, canon), fb.lay.n_used, st) != hipSuccess) return err_hip(err, "copy positions");
+    int rc = ensure_engine(fb, canon, st, err);
+    if (rc) return rc;
+    if (launch_copy_prev(fb.eng[fb.cur], fb.lay.n_used, st) != hipSuccess) return err_hip(err, "copy positions");
     fb.drifted = false;
-    int rc = launch_tiles_step(fb, canon, true, st, err);
-    if (!rc) rc = launch_ball_step(fb, canon, true, false, false, st, err);
+    fb.canon_valid = false;
+    rc = launch_bounds(fb, st, err);
+    if (!rc) rc = launch_tiles_step(fb, true, st, err);
+    if (!rc) rc = launch_ball_step(fb, true, false, false, st, err);
     if (!rc) rc = read_energies(fb, e_sys, st, err);
-    if (!rc) rc = fast_check(fb, st, err);
     return rc;
 }
 
 int fast_steps(FastBuffers& fb, State& canon, int n_steps, long long* step_count, int write_every,
                sl_energy* e_writes, int cap, hipStream_t st, std::string* err)
 {
+    int rc = ensure_engine(fb, canon, st, err);
+    if (rc) return rc;
+    if (n_steps > 0) fb.canon_valid = false;
     const int ns = fb.lay.n_sys;
     const bool graphs = fb.use_graphs && !fb.hook.before;
     int nw = 0;
     int k = 0;
     auto record = [&](void) -> int {
         sl_energy* dst = (e_writes && nw < cap) ? e_writes + (size_t)nw * ns : nullptr;
-        const int rc = read_energies(fb, dst, st, err);
+        const int r = read_energies(fb, dst, st, err);
         nw++;
-        return rc;
+        return r;
     };
     while (k < n_steps) {
         const long long s = *step_count + 1;
@@ -559,50 +704,48 @@ int fast_steps(FastBuffers& fb, State& canon, int n_steps, long long* step_count
         const int seg = std::min(std::min(until, n_steps - 1 - k), GRAPH_MAX_STEPS);
         if (graphs && fb.drifted && split_ready(fb) && seg >= 2) {
             const bool write_last = write_every > 0 && ((s + seg - 1) % write_every) == 0;
-            int rc = graph_segment(fb, canon, seg, write_last, st, err);
+            rc = graph_segment(fb, seg, write_last, st, err);
             if (rc) return rc;
             *step_count += seg;
             k += seg;
             if (write_last && (rc = record())) return rc;
             continue;
         }
-        int rc = direct_step(fb, canon, write, k + 1 < n_steps, st, err);
+        rc = direct_step(fb, write, k + 1 < n_steps, st, err);
         if (rc) return rc;
         *step_count = s;
         k++;
         if (write && (rc = record())) return rc;
     }
     if (hipStreamSynchronize(st) != hipSuccess) return err_hip(err, "synchronize");
-    const int rc = fast_check(fb, st, err);
-    if (rc) return rc;
     return nw;
 }
 
-int fast_check(FastBuffers& fb, hipStream_t st, std::string* err)
+hipError_t fast_reset_counters(FastBuffers& fb, hipStream_t st)
 {
-    int h[2] = {0, 0};
-    if (hipMemcpyAsync(h, fb.flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-        return err_hip(err, "read flags");
-    if (h[0]) {
-        if (err)
-            *err = "near-pair list overflow: a sphere has more than 32 partners within h_min of "
-                   "contact; results of this call are invalid";
-        return SL_E_CAPACITY;
-    }
-    return 0;
+    if (!fb.tile_ctr) return hipSuccess;
+    return launch_zero_u64(fb.tile_ctr, 2 * fb.cap_tiles, st);
 }
-
-hipError_t fast_reset_counters(FastBuffers& fb, hipStream_t st) { return launch_zero_u64(fb.counters, 2, st); }
 
 int fast_read_counters(FastBuffers& fb, hipStream_t st, long long* contacts, long long* near_pairs)
 {
-    unsigned long long h[2] = {0, 0};
-    if (hipMemcpyAsync(h, fb.counters, sizeof(h), hipMemcpyDeviceToHost, st) != hipSuccess ||
+    std::vector<unsigned long long> h(2 * (size_t)fb.n_tiles);
+    if (h.empty()) {
+        if (contacts) *contacts = 0;
+        if (near_pairs) *near_pairs = 0;
+        return 0;
+    }
+    if (hipMemcpyAsync(h.data(), fb.tile_ctr, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         return SL_E_HIP;
-    if (contacts) *contacts = (long long)h[0];
-    if (near_pairs) *near_pairs = (long long)(h[1] / 2);   // each near pair is filed with both balls
+    long long cc = 0, cn = 0;
+    for (size_t b = 0; b < h.size(); b += 2) {
+        cc += (long long)h[b];
+        cn += (long long)h[b + 1];
+    }
+    if (contacts) *contacts = cc;
+    if (near_pairs) *near_pairs = cn;
     return 0;
 }
 

@@ -61,9 +61,11 @@ def test_fast_1000_vs_oracle(oracle_mod, far_gravity, v_sigma):
     first 20 steps; afterwards any rounding difference grows with an e-folding
     time of about five steps (the loading/unloading spring switch at maximum
     compression is discontinuous), reaching ~1e-14 in positions and ~1e-11 in
-    velocities at step 100 and O(1) by step ~500.  Only the ordered path, which
-    reproduces the reference's rounding exactly, stays on the reference
-    trajectory beyond that."""
+    velocities at step 100 and O(1) by step ~500.  Spins, driven by the friction
+    terms whose direction switches discontinuously, drift fastest (~5e-10 at step
+    100), so their bar there is 1e-9; the one-step bars on forces and torques are
+    what pin the fast arithmetic itself.  Only the ordered path, which reproduces
+    the reference's rounding exactly, stays on the reference trajectory beyond that."""
     n = 1000
     g = sl.generate_bpca(n, seed=0x5EED0000, v_sigma=v_sigma)
     base = sl.preset("config" if far_gravity else "reference")
@@ -73,14 +75,16 @@ def test_fast_1000_vs_oracle(oracle_mod, far_gravity, v_sigma):
     out1, e0, _ = _run(g, p, 1)
     s1, oe0, _ = oracle_mod.run(st, P, 2, 0, mode="gather", nthreads=NTHREADS)
     assert relerr(out1.acc, s1.acc) <= 1e-13
+    assert relerr(out1.aacc, s1.aacc) <= 1e-13
     assert relerr(out1.vel, s1.vel) <= 1e-14
+    assert relerr(out1.w, s1.w) <= 1e-13
     assert relerr(out1.pos, s1.pos) <= 1e-15
     assert abs(e0.PE - oe0[0]) <= 1e-12 * abs(oe0[0])
     out, _, ew = _run(g, p, 100, 100)
     s, _, oew = oracle_mod.run(st, P, 101, 100, mode="gather", nthreads=NTHREADS)
     assert relerr(out.pos, s.pos) <= 1e-12
     assert relerr(out.vel, s.vel) <= 1e-10
-    assert relerr(out.w, s.w) <= 1e-10 or np.max(np.abs(s.w)) == 0
+    assert relerr(out.w, s.w) <= 1e-9 or np.max(np.abs(s.w)) == 0
     scale = abs(oew[-1][0]) + oew[-1][1]
     assert abs(ew[-1].E - (oew[-1][0] + oew[-1][1])) <= 1e-12 * scale
 
@@ -183,34 +187,38 @@ def test_mode_switch_continues_state(oracle_mod):
     assert relerr(out.vel, s.vel) <= 1e-12
 
 
-def test_near_list_overflow_is_an_error():
-    """A large sphere with 40 small spheres touching it exceeds the 32-entry near list."""
+def test_many_partners_per_sphere(oracle_mod):
+    """A large sphere pressed against by 150 small ones: more near partners than one
+    gather chunk of the per-ball kernel (64), all of them touching."""
     rng = np.random.default_rng(3)
-    k = 40
-    Rb, Rs = 1e-4, 1e-5
+    k = 150
+    Rb, Rs = 2e-4, 1e-5
     d = rng.standard_normal((k, 3))
     d /= np.linalg.norm(d, axis=1)[:, None]
-    # spread the small spheres so they do not overlap each other
-    for _ in range(200):
-        for a in range(k):
-            diff = d[a] - d
-            dist = np.linalg.norm(diff, axis=1)
-            dist[a] = 1
-            d[a] += 0.05 * (diff / dist[:, None] ** 3).sum(0)
-            d[a] /= np.linalg.norm(d[a])
-    pos = np.vstack([[0, 0, 0], d * (Rb + Rs)])
+    # spread the small spheres over the surface so they do not overlap each other
+    for _ in range(300):
+        diff = d[:, None, :] - d[None, :, :]
+        dist = np.linalg.norm(diff, axis=2) + np.eye(k)
+        d += 0.002 * (diff / dist[:, :, None] ** 3).sum(1)
+        d /= np.linalg.norm(d, axis=1)[:, None]
+    ang = np.arccos(np.clip(d @ d.T - 2 * np.eye(k), -1, 1))
+    assert ang.min() * (Rb + Rs) > 2.2 * Rs
+    pos = np.vstack([[0, 0, 0], d * (Rb + Rs) * (1 - 1e-3)])
     R = np.array([Rb] + [Rs] * k)
     m = 2.25 * 4 / 3 * np.pi * R ** 3
-    g = sl.BallGroup(pos, np.zeros_like(pos), np.zeros_like(pos), R, m, 0.4 * m * R * R)
+    vel = rng.standard_normal(pos.shape) * 1e-3
+    g = sl.BallGroup(pos, vel, np.zeros_like(pos), R, m, 0.4 * m * R * R)
     p = sl.calibrate(sl.preset("reference"), R, m, velocity=V_CUSTOM, cor=0.64).replace(
         order=sl.ORDER_FAST)
-    with sl.Collider(len(R), p) as c:
-        c.upload(g)
-        with pytest.raises(sl.SpacelabError) as ei:
-            c.init_conditions()
-        assert ei.value.code == -4
-    # the ordered path has no list and runs the same input
-    with sl.Collider(len(R), p.replace(order=sl.ORDER_ORDERED)) as c:
-        c.upload(g)
-        c.init_conditions()
-        c.step(2)
+    st = oracle_mod.State(g.pos, g.vel, g.w, g.R, g.m, g.moi)
+    P = _oracle_params(oracle_mod, p)
+    out1, e0, _ = _run(g, p, 1)
+    s1, oe0, _ = oracle_mod.run(st, P, 2, 0, mode="gather", nthreads=NTHREADS)
+    assert e0.contacts == k
+    assert relerr(out1.acc, s1.acc) <= 1e-13
+    assert relerr(out1.vel, s1.vel) <= 1e-14
+    assert abs(e0.PE - oe0[0]) <= 1e-12 * abs(oe0[0])
+    out, _, _ = _run(g, p, 20)
+    s, _, _ = oracle_mod.run(st, P, 21, 0, mode="gather", nthreads=NTHREADS)
+    assert relerr(out.pos, s.pos) <= 1e-12
+    assert relerr(out.vel, s.vel) <= 1e-10
