@@ -192,6 +192,93 @@ int sl_gen_bpca(int n, unsigned long long seed, double r_med, double sigma, doub
                 double v_sigma, double* pos3, double* vel3, double* w3, double* R, double* m,
                 double* moi);
 
+/* ---- collisions ------------------------------------------------------------- */
+/* Two-aggregate collision (the two-cluster set-up of ball_group.hpp:1368-1442, with
+ * the impact parameter as a distance, no reference counterpart for the rest): a
+ * target of n_t spheres and a projectile of n_p spheres, each a seeded aggregate as
+ * sl_gen_bpca builds it (projectile seed derived from seed) with no internal motion,
+ * each in a uniformly random orientation.  The projectile's centre of mass sits at
+ * (-(R_t + R_p + gap * r_max), b, 0) with b = b_frac * (R_t + R_p), R_t and R_p the
+ * bodies' outer radii; it moves along +x at `speed` relative to the target and the
+ * total momentum is zero.  Arrays hold n_t + n_p entries, target first.  geom (may be
+ * NULL) receives {R_t, R_p, b, M_t, M_p, r_max}. */
+int sl_gen_collision(int n_t, int n_p, unsigned long long seed, double speed, double b_frac, double gap,
+                     double r_med, double sigma, double rho, double* pos3, double* vel3, double* w3,
+                     double* R, double* m, double* moi, double* geom);
+
+/* Outcome of a finished collision from its contact graph: spheres closer than
+ * R_i + R_j + link are bound.  With C1, C2 the two heaviest components:
+ *   stick    C1 holds at least half of each body's mass and 90% of the total;
+ *   bounce   otherwise, if C1 and C2 hold at least half of one body each and all
+ *            other components together less than 10% of the mass;
+ *   fragment otherwise. */
+#define SL_OUTCOME_STICK 0
+#define SL_OUTCOME_BOUNCE 1
+#define SL_OUTCOME_FRAGMENT 2
+typedef struct {
+    int label;                /* SL_OUTCOME_* */
+    int n_components;
+    double largest_mass_frac; /* mass of C1 / total mass */
+    double second_mass_frac;  /* mass of C2 / total mass */
+    double mass_transfer;     /* projectile mass in the component holding most of the
+                                 target's mass, / projectile mass */
+    double v_largest[3];      /* centre-of-mass velocity of C1 */
+    double v_second[3];       /* centre-of-mass velocity of C2 (0 if none) */
+} sl_outcome;
+int sl_collision_outcome(int n_t, int n_p, const double* pos3, const double* vel3, const double* R,
+                         const double* m, double link, sl_outcome* out);
+
+/* Moves the projectile (balls n_t..) along +x so that its first contact with the
+ * target comes after `gap` (cm) of travel.  Returns 1 and the shift applied (cm) when
+ * the bodies meet on that line, 0 with no shift when they miss, < 0 on bad input. */
+int sl_collision_approach(int n_t, int n_p, double* pos3, const double* R, double gap, double* shift);
+
+/* Descriptors of a collision set-up (inputs of the surrogate model), SL_N_DESCRIPTORS
+ * values:
+ *   0-3   target {ln N, radius of gyration / mean radius, mass-radius fractal dimension
+ *         ln N / ln(Rg/r), mean coordination (neighbours closer than R_i + R_j + link)}
+ *   4-7   the same for the projectile
+ *   8-11  log10 speed, b_frac, projectile mass fraction, log10 kinetic energy in the
+ *         centre-of-mass frame
+ *   12    1 if the bodies meet moving along +x, else 0
+ *   13-14 mass share of the projectile in the target's shadow along x, and of the
+ *         target in the projectile's
+ *   15    ln(1 + target balls crossed per projectile mass unit along x)
+ *   16-20 at the first contact (0 on a miss): cosine of the contact normal with the
+ *         approach line, distance of the contact point from the target's centre of
+ *         mass / R_t and from the projectile's / R_p, and the neighbour counts of the
+ *         two touching balls within their own bodies */
+#define SL_N_DESCRIPTORS 21
+int sl_collision_descriptors(int n_t, int n_p, const double* pos3, const double* R, const double* m,
+                             double speed, double b_frac, double link, double* desc);
+
+/* ---- surrogate model (no reference counterpart) ---------------------------
+ * A 4-layer perceptron n_in -> width -> width -> width -> n_out with ReLU between
+ * layers, trained and evaluated on the GPU; its matrix products use bfloat16
+ * operands on the matrix cores with binary32 accumulation, binary32 master weights
+ * and Adam.  Outputs 0..2 are the logits of {stick, bounce, fragment}; outputs 3..
+ * are regression targets.  Loss per sample: softmax cross-entropy + mean squared
+ * error of the regression outputs; a batch's loss is the mean over its samples.
+ * Parameters are exchanged flat: for each layer W[in][out] (row-major) then b[out].
+ * Every reduction has a fixed order, so training is reproducible bit for bit. */
+typedef struct sl_mlp sl_mlp;
+int sl_mlp_create(sl_mlp** out, int n_in, int width, int n_out, int max_batch, unsigned long long seed);
+int sl_mlp_n_params(const sl_mlp* m);
+int sl_mlp_get_params(sl_mlp* m, float* params);
+int sl_mlp_set_params(sl_mlp* m, const float* params);   /* also resets the Adam state */
+/* y: batch x n_out outputs; prob: batch x 3 class probabilities; ent: predictive
+ * entropy (nats) of each sample's class distribution.  Any of them may be NULL. */
+int sl_mlp_forward(sl_mlp* m, int batch, const float* x, float* y, float* prob, float* ent);
+/* Mean loss of a labelled batch and (if grads is not NULL) its gradient with respect
+ * to every parameter, without updating them. */
+int sl_mlp_gradients(sl_mlp* m, int batch, const float* x, const int* cls, const float* reg, float* loss,
+                     float* grads);
+/* One Adam step (beta 0.9, 0.999, eps 1e-8) on a labelled batch; loss before the step. */
+int sl_mlp_train_step(sl_mlp* m, int batch, const float* x, const int* cls, const float* reg, float lr,
+                      float* loss);
+const char* sl_mlp_last_error(const sl_mlp* m);
+void sl_mlp_destroy(sl_mlp* m);
+
 /* ---- arithmetic probe ------------------------------------------------------ */
 /* q = a/b, s = sqrt(a), lg = log(a) element-wise on the device, compiled exactly as
  * the ordered kernels are.  Lets tests check that binary64 division and square

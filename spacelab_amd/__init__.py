@@ -64,6 +64,17 @@ class _Energy(C.Structure):
                 ("L", C.c_double * 3), ("nonfinite", C.c_int), ("contacts", C.c_longlong)]
 
 
+class Outcome(C.Structure):
+    """sl_outcome: contact-graph outcome of a finished collision."""
+    _fields_ = [("label", C.c_int), ("n_components", C.c_int), ("largest_mass_frac", C.c_double),
+                ("second_mass_frac", C.c_double), ("mass_transfer", C.c_double),
+                ("v_largest", C.c_double * 3), ("v_second", C.c_double * 3)]
+
+
+OUTCOME_STICK, OUTCOME_BOUNCE, OUTCOME_FRAGMENT = 0, 1, 2
+N_DESCRIPTORS = 21
+
+
 class _Profile(C.Structure):
     _fields_ = [("pair_ms", C.c_double), ("pair_launches", C.c_longlong),
                 ("ball_ms", C.c_double), ("ball_launches", C.c_longlong),
@@ -144,6 +155,24 @@ def lib():
     L.sl_batch_destroy.restype = None
     L.sl_gen_bpca.argtypes = [C.c_int, C.c_ulonglong, C.c_double, C.c_double, C.c_double,
                               C.c_double] + [dp] * 6
+    L.sl_gen_collision.argtypes = [C.c_int, C.c_int, C.c_ulonglong, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_double] + [dp] * 7
+    L.sl_collision_outcome.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, C.c_double, C.POINTER(Outcome)]
+    L.sl_collision_approach.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, dp]
+    L.sl_collision_descriptors.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_double,
+                                           C.c_double, dp]
+    fp = C.POINTER(C.c_float)
+    L.sl_mlp_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong]
+    L.sl_mlp_n_params.argtypes = [vp]
+    L.sl_mlp_get_params.argtypes = [vp, fp]
+    L.sl_mlp_set_params.argtypes = [vp, fp]
+    L.sl_mlp_forward.argtypes = [vp, C.c_int, fp, fp, fp, fp]
+    L.sl_mlp_gradients.argtypes = [vp, C.c_int, fp, ip, fp, fp, fp]
+    L.sl_mlp_train_step.argtypes = [vp, C.c_int, fp, ip, fp, C.c_float, fp]
+    L.sl_mlp_last_error.argtypes = [vp]
+    L.sl_mlp_last_error.restype = C.c_char_p
+    L.sl_mlp_destroy.argtypes = [vp]
+    L.sl_mlp_destroy.restype = None
     L.sl_probe_divsqrt.argtypes = [C.c_int] + [dp] * 5
     L.sl_probe_rotation.argtypes = [C.POINTER(C.c_int)]
     L.sl_probe_copy.argtypes = [C.c_ulonglong, C.c_int, C.POINTER(C.c_double)]
@@ -221,6 +250,73 @@ def generate_bpca(n: int, seed: int, r_med: float = 1e-5, sigma: float = 0.2,
     if rc:
         raise SpacelabError(rc, "sl_gen_bpca")
     return BallGroup(pos, vel, w, R, m, moi)
+
+
+@dataclass
+class CollisionSetup:
+    """A two-aggregate collision (sl_gen_collision): the combined group (target
+    first) and its geometry."""
+    group: BallGroup
+    n_t: int
+    n_p: int
+    speed: float
+    b_frac: float
+    R_t: float
+    R_p: float
+    b: float
+    M_t: float
+    M_p: float
+    r_max: float
+
+
+def generate_collision(n_t: int, n_p: int, seed: int, speed: float, b_frac: float, gap: float = 2.0,
+                       r_med: float = 1e-5, sigma: float = 0.2, rho: float = 2.25) -> CollisionSetup:
+    """Target and projectile aggregates aimed at each other (see sl_gen_collision)."""
+    n = n_t + n_p
+    pos = np.zeros((n, 3)); vel = np.zeros((n, 3)); w = np.zeros((n, 3))
+    R = np.zeros(n); m = np.zeros(n); moi = np.zeros(n); geom = np.zeros(6)
+    rc = lib().sl_gen_collision(n_t, n_p, seed, speed, b_frac, gap, r_med, sigma, rho, _dp(pos),
+                                _dp(vel), _dp(w), _dp(R), _dp(m), _dp(moi), _dp(geom))
+    if rc:
+        raise SpacelabError(rc, "sl_gen_collision")
+    return CollisionSetup(BallGroup(pos, vel, w, R, m, moi), n_t, n_p, speed, b_frac, *geom)
+
+
+def collision_approach(setup: CollisionSetup, gap: float) -> tuple[bool, float]:
+    """Move the projectile along +x (in place) so that its first contact with the
+    target comes after ``gap`` cm of travel; (met, shift), met False on a miss."""
+    g = setup.group
+    n = g.num_particles
+    g.pos = _f64(g.pos, (n, 3))
+    sh = C.c_double()
+    rc = lib().sl_collision_approach(setup.n_t, setup.n_p, _dp(g.pos), _dp(_f64(g.R, (n,))), gap, C.byref(sh))
+    if rc < 0:
+        raise SpacelabError(rc, "sl_collision_approach")
+    return bool(rc), float(sh.value)
+
+
+def collision_outcome(n_t: int, g: BallGroup, link: float) -> Outcome:
+    """Contact-graph outcome (stick / bounce / fragment) of a collision's final state."""
+    n = g.num_particles
+    pos, vel = _f64(g.pos, (n, 3)), _f64(g.vel, (n, 3))
+    R, m = _f64(g.R, (n,)), _f64(g.m, (n,))
+    o = Outcome()
+    rc = lib().sl_collision_outcome(n_t, n - n_t, _dp(pos), _dp(vel), _dp(R), _dp(m), link, C.byref(o))
+    if rc:
+        raise SpacelabError(rc, "sl_collision_outcome")
+    return o
+
+
+def collision_descriptors(setup: CollisionSetup, link: float) -> np.ndarray:
+    """The N_DESCRIPTORS inputs of the surrogate model for a collision set-up."""
+    g = setup.group
+    n = g.num_particles
+    out = np.zeros(N_DESCRIPTORS)
+    rc = lib().sl_collision_descriptors(setup.n_t, setup.n_p, _dp(_f64(g.pos, (n, 3))), _dp(_f64(g.R, (n,))),
+                                        _dp(_f64(g.m, (n,))), setup.speed, setup.b_frac, link, _dp(out))
+    if rc:
+        raise SpacelabError(rc, "sl_collision_descriptors")
+    return out
 
 
 class Collider:
@@ -458,6 +554,96 @@ class Batch:
         self._check(lib().sl_batch_get_option(self._b, _OPTS[name], C.byref(v)),
                     "sl_batch_get_option")
         return v.value
+
+
+def _fp(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class MLP:
+    """The surrogate perceptron on the GPU (sl_mlp_*): n_in -> width x 3 -> n_out."""
+
+    def __init__(self, n_in: int, n_out: int, width: int = 256, max_batch: int = 4096, seed: int = 1):
+        self._m = C.c_void_p()
+        rc = lib().sl_mlp_create(C.byref(self._m), n_in, width, n_out, max_batch, seed)
+        if rc:
+            raise SpacelabError(rc, "sl_mlp_create")
+        self.n_in, self.n_out, self.width, self.max_batch = n_in, n_out, width, max_batch
+
+    def _check(self, rc: int, what: str) -> int:
+        if rc < 0:
+            msg = lib().sl_mlp_last_error(self._m)
+            raise SpacelabError(rc, f"{what}: {msg.decode() if msg else ''}")
+        return rc
+
+    def close(self):
+        if self._m:
+            lib().sl_mlp_destroy(self._m)
+            self._m = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def n_params(self) -> int:
+        return int(lib().sl_mlp_n_params(self._m))
+
+    def get_params(self) -> np.ndarray:
+        w = np.zeros(self.n_params, dtype=np.float32)
+        self._check(lib().sl_mlp_get_params(self._m, _fp(w)), "sl_mlp_get_params")
+        return w
+
+    def set_params(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.shape == (self.n_params,)
+        self._check(lib().sl_mlp_set_params(self._m, _fp(w)), "sl_mlp_set_params")
+
+    def forward(self, x: np.ndarray):
+        """(outputs, class probabilities, predictive entropy) of a batch."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        B = x.shape[0]
+        y = np.zeros((B, self.n_out), np.float32); p = np.zeros((B, 3), np.float32)
+        e = np.zeros(B, np.float32)
+        self._check(lib().sl_mlp_forward(self._m, B, _fp(x), _fp(y), _fp(p), _fp(e)), "sl_mlp_forward")
+        return y, p, e
+
+    def gradients(self, x, cls, reg):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        cls = np.ascontiguousarray(cls, dtype=np.int32)
+        reg = np.ascontiguousarray(reg, dtype=np.float32)
+        loss = C.c_float()
+        g = np.zeros(self.n_params, np.float32)
+        self._check(lib().sl_mlp_gradients(self._m, x.shape[0], _fp(x), _ip(cls), _fp(reg), C.byref(loss), _fp(g)),
+                    "sl_mlp_gradients")
+        return float(loss.value), g
+
+    def train_step(self, x, cls, reg, lr: float) -> float:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        cls = np.ascontiguousarray(cls, dtype=np.int32)
+        reg = np.ascontiguousarray(reg, dtype=np.float32)
+        loss = C.c_float()
+        self._check(lib().sl_mlp_train_step(self._m, x.shape[0], _fp(x), _ip(cls), _fp(reg), lr, C.byref(loss)),
+                    "sl_mlp_train_step")
+        return float(loss.value)
 
 
 def probe_rotation() -> np.ndarray:

@@ -22,7 +22,7 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libspacelab.so")
 
 SOURCES = ["sl_api.hip", "sl_kernels_ordered.hip", "sl_kernels_fast.hip", "sl_util.hip",
-           "sl_batch.hip", "sl_mlp.hip", "sl_host.cpp"]
+           "sl_batch.hip", "sl_mlp.hip", "sl_host.cpp", "sl_collide.cpp"]
 HEADERS = ["sl_common.hpp", "sl_pair_ref.hpp", "sl_internal.hpp", "sl_fast_pair.hpp",
            "sl_fast_kernels.hpp"]
 

@@ -1,0 +1,391 @@
+"""Learned collision surrogate: DEM-labelled collision datasets from batched
+ensembles, the GPU perceptron (:class:`spacelab_amd.MLP`) trained on them, and
+entropy-gated inference that hands every collision whose predictive entropy is high
+back to DEM.
+
+The reference has no counterpart: it simulates every collision with DEM, one
+sim_one_step per time step (ColliderSingleCore.cpp:78-288).  Here DEM is the judge of
+the surrogate: its labels train the network and decide every collision the network
+is unsure of.
+
+Collision set-ups follow the two-cluster arrangement (ball_group.hpp:1368-1442):
+target and projectile aggregates from the seeded BPCA generator, random orientations,
+impact parameter b_frac * (R_t + R_p), zero total momentum.  The projectile is moved
+along the line of approach until its first contact is ``gap`` median radii away, so
+DEM steps are spent on the collision rather than the approach.  Springs are calibrated
+at the impact speed by updateDTK's rule (ball_group.hpp:1313-1329) and divided by
+``softening``.  dust_const.hpp:35's kConsts carries a factor 3e3 that the kConsts of
+asteroid_const.hpp:25 and ColliderParallel/structures.hpp:1106 lack; a softening s
+takes sqrt(s) times fewer steps per radius of travel (t_c = pi sqrt(m/k), dt = t_c/50).
+The deepest overlap of an impact grows as sqrt(s N_p) (the projectile's N_p balls
+press on the first contacts), so s stays well below 3e3: at s = 3e3 a 64-ball
+projectile at 1000 cm/s drives balls through one another.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import time
+from dataclasses import asdict, dataclass, field
+
+import numpy as np
+
+from . import (N_DESCRIPTORS, OUTCOME_BOUNCE, OUTCOME_FRAGMENT, OUTCOME_STICK, ORDER_FAST,
+               PREC_FP64, Batch, CollisionSetup, MLP, calibrate, collision_approach,
+               collision_descriptors, collision_outcome, generate_collision, preset)
+
+CLASS_NAMES = ("stick", "bounce", "fragment")
+# regression targets: mass fraction of the largest body, share of the projectile's mass
+# that ends in the target's body, and the centre-of-mass velocities (x, y) of the two
+# largest bodies divided by the impact speed (zero-momentum frame)
+REG_NAMES = ("largest_mass_frac", "mass_transfer", "v1x", "v1y", "v2x", "v2y")
+N_REG = len(REG_NAMES)
+
+
+@dataclass(frozen=True)
+class Spec:
+    """One collision: body sizes, generator seed, impact speed (cm/s), b / (R_t + R_p)."""
+    n_t: int
+    n_p: int
+    seed: int
+    speed: float
+    b_frac: float
+
+
+@dataclass
+class DemSettings:
+    steps: int = 40000          # DEM steps after set-up (same for every collision)
+    gap: float = 0.05           # travel before first contact, median radii
+    link: float = 0.1           # outcome link distance, median radii
+    r_med: float = 1e-5         # median ball radius, cm
+    sigma: float = 0.2          # lognormal width of the radii
+    softening: float = 100.0    # divisor of the calibrated spring constants
+    precision: int = PREC_FP64
+    max_systems: int = 512      # collisions per batched launch
+    preset: str = "config"
+
+
+def _log_uniform_int(rng, lo, hi, n):
+    return np.minimum(np.floor(np.exp(rng.uniform(math.log(lo), math.log(hi + 1), n))), hi).astype(int)
+
+
+def sample_specs(count: int, seed: int, n_t=(32, 512), n_p=(1, 64), speed=(10.0, 1000.0)) -> list[Spec]:
+    """Log-uniform body sizes, uniform speed and impact parameter, per-collision seeds."""
+    rng = np.random.default_rng(seed)
+    nt = _log_uniform_int(rng, *n_t, count)
+    npj = _log_uniform_int(rng, *n_p, count)
+    v = rng.uniform(*speed, count)
+    b = rng.uniform(0.0, 1.0, count)
+    seeds = rng.integers(1, 2 ** 62, count)
+    return [Spec(int(a), int(c), int(s), float(x), float(y)) for a, c, s, x, y in zip(nt, npj, seeds, v, b)]
+
+
+@dataclass
+class Prepared:
+    setup: CollisionSetup
+    kin: float
+    kout: float
+    dt: float
+    met: bool
+
+
+def prepare(spec: Spec, s: DemSettings) -> Prepared:
+    setup = generate_collision(spec.n_t, spec.n_p, spec.seed, spec.speed, spec.b_frac, gap=2.0,
+                               r_med=s.r_med, sigma=s.sigma)
+    met, _ = collision_approach(setup, s.gap * s.r_med)
+    g = setup.group
+    p = calibrate(preset(s.preset), g.R, g.m, velocity=spec.speed, cor=0.64, dt_rule="tc50")
+    kin, kout, dt = p.kin, p.kout, p.dt
+    if s.softening != 1.0:
+        # t_c = pi sqrt(m / k): dividing k by s multiplies t_c/50 by sqrt(s)
+        kin, kout, dt = kin / s.softening, kout / s.softening, dt * math.sqrt(s.softening)
+    return Prepared(setup, kin, kout, dt, met)
+
+
+def descriptors(preps: list[Prepared], s: DemSettings) -> np.ndarray:
+    return np.stack([collision_descriptors(q.setup, s.link * s.r_med) for q in preps]).astype(np.float32)
+
+
+def targets(o, speed: float) -> np.ndarray:
+    return np.array([o.largest_mass_frac, o.mass_transfer, o.v_largest[0] / speed, o.v_largest[1] / speed,
+                     o.v_second[0] / speed, o.v_second[1] / speed])
+
+
+class DemRunner:
+    """Runs collisions to their outcome with the batched fast engine."""
+
+    def __init__(self, s: DemSettings, max_balls: int):
+        self.s = s
+        p = preset(s.preset).replace(order=ORDER_FAST, precision=s.precision)
+        self.batch = Batch(s.max_systems, max_balls, p)
+
+    def close(self):
+        self.batch.close()
+
+    def run(self, preps: list[Prepared], steps: int | None = None) -> tuple[np.ndarray, np.ndarray, list]:
+        """(class labels, regression targets, outcomes) of the collisions after
+        ``steps`` (default: the settings') DEM steps."""
+        s = self.s
+        steps = s.steps if steps is None else steps
+        labels, regs, outs = [], [], []
+        for c0 in range(0, len(preps), s.max_systems):
+            chunk = preps[c0:c0 + s.max_systems]
+            b = self.batch
+            b.upload([q.setup.group for q in chunk], kin=[q.kin for q in chunk], kout=[q.kout for q in chunk],
+                     dt=[q.dt for q in chunk])
+            b.init_conditions()
+            b.step(steps)
+            for q, g in zip(chunk, b.download()):
+                o = collision_outcome(q.setup.n_t, g, s.link * s.r_med)
+                labels.append(o.label)
+                regs.append(targets(o, q.setup.speed))
+                outs.append(o)
+        return np.array(labels, dtype=np.int32), np.array(regs, dtype=np.float32).reshape(-1, N_REG), outs
+
+
+@dataclass
+class Normaliser:
+    mean: np.ndarray
+    std: np.ndarray
+
+    @staticmethod
+    def fit(X: np.ndarray) -> "Normaliser":
+        mu = X.mean(0)
+        sd = X.std(0)
+        return Normaliser(mu, np.where(sd > 0, sd, 1.0))
+
+    def __call__(self, X: np.ndarray) -> np.ndarray:
+        return np.ascontiguousarray((X - self.mean) / self.std, dtype=np.float32)
+
+
+@dataclass
+class Surrogate:
+    """An ensemble of perceptrons (different initialisations and batch orders) with
+    their input normalisation and the entropy gate.  The class distribution is the
+    mean of the members' softmax outputs and the gate reads its entropy (predictive
+    entropy of the ensemble)."""
+    width: int = 256
+    seed: int = 1
+    n_models: int = 4
+    max_batch: int = 4096
+    threshold: float = float("inf")
+    norm: Normaliser | None = None
+    mlps: list = field(default_factory=list, repr=False)
+
+    def __post_init__(self):
+        if not self.mlps:
+            self.mlps = [MLP(N_DESCRIPTORS, 3 + N_REG, self.width, self.max_batch, self.seed * 1000 + k)
+                         for k in range(self.n_models)]
+
+    def close(self):
+        for m in self.mlps:
+            m.close()
+        self.mlps = []
+
+    def fit(self, X, cls, reg, epochs=30, batch=256, lr=1e-3, seed=0, log=None) -> list[float]:
+        """Adam on shuffled mini-batches with a cosine step-size decay; returns the
+        members' mean loss of each epoch."""
+        self.norm = Normaliser.fit(X)
+        Xn = self.norm(X)
+        cls = np.ascontiguousarray(cls, dtype=np.int32)
+        reg = np.ascontiguousarray(reg, dtype=np.float32)
+        n = len(Xn)
+        hist = np.zeros(epochs)
+        for k, m in enumerate(self.mlps):
+            rng = np.random.default_rng(seed * 1000 + k)
+            for ep in range(epochs):
+                perm = rng.permutation(n)
+                tot, cnt = 0.0, 0
+                lr_ep = lr * 0.5 * (1 + math.cos(math.pi * ep / epochs))
+                for b0 in range(0, n - batch + 1, batch):
+                    idx = perm[b0:b0 + batch]
+                    tot += m.train_step(Xn[idx], cls[idx], reg[idx], lr_ep) * len(idx)
+                    cnt += len(idx)
+                hist[ep] += tot / max(cnt, 1) / len(self.mlps)
+        if log:
+            log(f"loss by epoch: {np.round(hist, 4).tolist()}")
+        return hist.tolist()
+
+    def predict(self, X):
+        """(class, regression outputs, class probabilities, predictive entropy) of
+        descriptor rows."""
+        Xn = self.norm(X)
+        P = np.zeros((len(Xn), 3))
+        Y = np.zeros((len(Xn), N_REG))
+        for m in self.mlps:
+            for k in range(0, len(Xn), self.max_batch):
+                y, p, _ = m.forward(Xn[k:k + self.max_batch])
+                P[k:k + len(p)] += p
+                Y[k:k + len(y)] += y[:, 3:]
+        P /= len(self.mlps)
+        Y /= len(self.mlps)
+        ent = -(P * np.log(np.where(P > 0, P, 1.0))).sum(1)
+        return P.argmax(1).astype(np.int32), Y.astype(np.float32), P, ent
+
+    def calibrate_gate(self, X, cls, target: float) -> dict:
+        """Smallest fallback share whose gated agreement on (X, cls) reaches ``target``:
+        collisions above the entropy threshold go to DEM, which is right by definition."""
+        pred, _, _, ent = self.predict(X)
+        order = np.argsort(-ent, kind="stable")
+        wrong = (pred != cls)[order]
+        n = len(cls)
+        # after sending the k most uncertain to DEM, the errors left are the wrong ones among the rest
+        left = wrong[::-1].cumsum()[::-1]           # left[k] = errors among order[k:]
+        agree = 1.0 - np.append(left, 0) / n       # agreement when k = 0..n go to DEM
+        k = int(np.argmax(agree >= target)) if np.any(agree >= target) else n
+        # the gate is ent > threshold: the k most uncertain go to DEM (ties with the
+        # (k+1)-th stay with the surrogate)
+        self.threshold = float(ent[order[k]]) if k < n else -1.0
+        fb = ent > self.threshold
+        final = np.where(fb, cls, pred)
+        return {"threshold": self.threshold, "fallback_share": float(fb.mean()),
+                "agreement": float((final == cls).mean()), "raw_accuracy": float((pred == cls).mean())}
+
+
+def run_gated(preps: list[Prepared], sur: Surrogate, dem: DemRunner, s: DemSettings):
+    """Outcome classes and regression targets for every collision: the surrogate's,
+    except where its entropy exceeds the gate, where DEM's.  Returns (cls, reg,
+    fallback mask, timings in seconds: descriptors, inference, DEM, total)."""
+    t0 = time.perf_counter()
+    X = descriptors(preps, s)
+    t1 = time.perf_counter()
+    cls, reg, _, ent = sur.predict(X)
+    fb = ent > sur.threshold
+    t2 = time.perf_counter()
+    idx = np.flatnonzero(fb)
+    if len(idx):
+        c, r, _ = dem.run([preps[i] for i in idx])
+        cls[idx] = c
+        reg[idx] = r
+    t3 = time.perf_counter()
+    return cls, reg, fb, {"descriptors": t1 - t0, "inference": t2 - t1, "dem": t3 - t2, "total": t3 - t0}
+
+
+def evaluate(train: dict, heldout: dict, target: float = 0.95, z: float = 2.0, val_frac: float = 0.2,
+             epochs: int = 40, lr: float = 1e-3, n_models: int = 4, seed: int = 1, log=None) -> dict:
+    """Train on ``train`` with a validation share held back to set the entropy gate,
+    then run the held-out collisions twice, by DEM alone and by the gated surrogate,
+    with live DEM as the judge of the second.  The gate aims at ``target`` plus z
+    binomial standard errors of an agreement measured on the validation share."""
+    s = settings_of(heldout)
+    n = len(train["cls"])
+    perm = np.random.default_rng(seed).permutation(n)
+    nv = int(n * val_frac)
+    va, tr = perm[:nv], perm[nv:]
+    X, C, Rg = train["X"], train["cls"], train["reg"]
+    sur = Surrogate(seed=seed, n_models=n_models)
+    margin = z * math.sqrt(target * (1 - target) / max(nv, 1))
+    dem = None
+    try:
+        t0 = time.perf_counter()
+        hist = sur.fit(X[tr], C[tr], Rg[tr], epochs=epochs, lr=lr, seed=seed, log=log)
+        t_train = time.perf_counter() - t0
+        gate = sur.calibrate_gate(X[va], C[va], min(target + margin, 1.0))
+        if log:
+            log(f"trained {epochs} epochs in {t_train:.1f} s, loss {hist[0]:.3f} -> {hist[-1]:.3f}; gate {gate}")
+        specs = specs_of(heldout)
+        preps = [prepare(q, s) for q in specs]
+        max_balls = s.max_systems * max(q.setup.group.num_particles for q in preps)
+        dem = DemRunner(s, max_balls)
+        dem.run(preps[:16], steps=256)          # first-use set-up (tile-split tuning, graphs)
+        t0 = time.perf_counter()
+        c_dem, r_dem, _ = dem.run(preps)
+        t_dem = time.perf_counter() - t0
+        c_g, r_g, fb, tg = run_gated(preps, sur, dem, s)
+        raw, _, _, _ = sur.predict(descriptors(preps, s))
+    finally:
+        sur.close()
+        if dem is not None:
+            dem.close()
+    nh = len(preps)
+    keep = ~fb
+    mse = float(((r_g[keep] - r_dem[keep]) ** 2).mean()) if keep.any() else 0.0
+    mse0 = float(((Rg[tr].mean(0) - r_dem[keep]) ** 2).mean()) if keep.any() else 0.0
+    c_d = t_dem / nh                                         # DEM seconds per collision, full batches
+    c_s = (tg["descriptors"] + tg["inference"]) / nh          # surrogate seconds per collision
+    return {
+        "n_train": int(len(tr)), "n_val": int(nv), "n_heldout": nh, "epochs": epochs, "n_models": n_models,
+        "train_s": t_train, "gate_target": target + margin,
+        "raw_accuracy": float((raw == c_dem).mean()), "agreement": float((c_g == c_dem).mean()),
+        "fallback_share": float(fb.mean()), "threshold": sur.threshold, "val_gate": gate,
+        "dem_s": t_dem, "gated_s": tg["total"], "gated_parts_s": tg,
+        "collisions_per_s_dem": nh / t_dem, "collisions_per_s_gated": nh / tg["total"],
+        "gain": t_dem / tg["total"],
+        # fallbacks of a long stream queue up into full batches: per collision, the
+        # surrogate's cost plus the fallback share of a full-batch DEM collision
+        "gain_streaming": c_d / (c_s + fb.mean() * c_d),
+        "reg_mse": mse, "reg_mse_mean_predictor": mse0,
+        "heldout_classes": np.bincount(c_dem, minlength=3).tolist(),
+        "dem_steps": s.steps, "softening": s.softening,
+    }
+
+
+N_T_RANGE, N_P_RANGE, SPEED_RANGE = (32, 512), (1, 64), (10.0, 1000.0)
+
+
+def make_dataset(count: int, seed: int, s: DemSettings, log=None) -> dict:
+    """DEM-labelled collisions: descriptors, classes, regression targets and the specs
+    that regenerate each set-up."""
+    specs = sample_specs(count, seed, N_T_RANGE, N_P_RANGE, SPEED_RANGE)
+    dem = DemRunner(s, s.max_systems * (N_T_RANGE[1] + N_P_RANGE[1]))
+    X, C, Rg, met = [], [], [], []
+    t0 = time.perf_counter()
+    try:
+        for c0 in range(0, count, s.max_systems):
+            preps = [prepare(q, s) for q in specs[c0:c0 + s.max_systems]]
+            X.append(descriptors(preps, s))
+            c, r, _ = dem.run(preps)
+            C.append(c); Rg.append(r); met += [q.met for q in preps]
+            if log:
+                log(f"{c0 + len(preps)}/{count} collisions, {time.perf_counter() - t0:.1f} s, "
+                    f"classes so far {np.bincount(np.concatenate(C), minlength=3).tolist()}")
+    finally:
+        dem.close()
+    return {"X": np.concatenate(X), "cls": np.concatenate(C), "reg": np.concatenate(Rg),
+            "met": np.array(met), "n_t": np.array([q.n_t for q in specs]),
+            "n_p": np.array([q.n_p for q in specs]), "seed": np.array([q.seed for q in specs], dtype=np.uint64),
+            "speed": np.array([q.speed for q in specs]), "b_frac": np.array([q.b_frac for q in specs]),
+            "settings": np.array(json.dumps(asdict(s))), "sample_seed": np.array(seed)}
+
+
+def redescribe(ds: dict) -> dict:
+    """The dataset with its descriptors recomputed from the stored specs (the set-ups
+    are regenerated exactly; the DEM labels are kept)."""
+    s = settings_of(ds)
+    out = dict(ds)
+    out["X"] = np.concatenate([descriptors([prepare(q, s) for q in chunk], s)
+                               for chunk in (specs_of(ds)[k:k + 512] for k in range(0, len(ds["cls"]), 512))])
+    return out
+
+
+def specs_of(ds: dict) -> list[Spec]:
+    return [Spec(int(a), int(b), int(c), float(d), float(e))
+            for a, b, c, d, e in zip(ds["n_t"], ds["n_p"], ds["seed"], ds["speed"], ds["b_frac"])]
+
+
+def settings_of(ds: dict) -> DemSettings:
+    return DemSettings(**json.loads(str(ds["settings"])))
+
+
+def load_dataset(path: str) -> dict:
+    with np.load(path, allow_pickle=False) as f:
+        return {k: f[k] for k in f.files}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="DEM-labelled collision datasets for the surrogate")
+    ap.add_argument("--count", type=int, default=20000)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=DemSettings.steps)
+    ap.add_argument("--softening", type=float, default=DemSettings.softening)
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args(argv)
+    s = DemSettings(steps=a.steps, softening=a.softening)
+    ds = make_dataset(a.count, a.seed, s, log=lambda m: print(m, flush=True))
+    np.savez_compressed(a.out, **ds)
+    print(f"wrote {a.out}: classes {np.bincount(ds['cls'], minlength=3).tolist()}", flush=True)
+
+
+if __name__ == "__main__":
+    main()
