@@ -262,12 +262,17 @@ def run_gated(preps: list[Prepared], sur: Surrogate, dem: DemRunner, s: DemSetti
     return cls, reg, fb, {"descriptors": t1 - t0, "inference": t2 - t1, "dem": t3 - t2, "total": t3 - t0}
 
 
-def evaluate(train: dict, heldout: dict, target: float = 0.95, z: float = 2.0, val_frac: float = 0.2,
+def evaluate(train: dict, heldout: dict, target: float = 0.95, z: float = 2.0, val_frac: float = 0.5,
              epochs: int = 40, lr: float = 1e-3, n_models: int = 4, seed: int = 1, log=None) -> dict:
     """Train on ``train`` with a validation share held back to set the entropy gate,
     then run the held-out collisions twice, by DEM alone and by the gated surrogate,
-    with live DEM as the judge of the second.  The gate aims at ``target`` plus z
-    binomial standard errors of an agreement measured on the validation share."""
+    with live DEM as the judge of the second.
+
+    The gate aims above ``target`` by z standard errors of the difference between two
+    binomial estimates, the validation share's and the held-out set's, so that the
+    held-out agreement clears ``target`` with probability about Phi(z).  Accuracy
+    grows little with the training set beyond a few thousand collisions, so half of
+    them go to validation by default."""
     s = settings_of(heldout)
     n = len(train["cls"])
     perm = np.random.default_rng(seed).permutation(n)
@@ -275,7 +280,8 @@ def evaluate(train: dict, heldout: dict, target: float = 0.95, z: float = 2.0, v
     va, tr = perm[:nv], perm[nv:]
     X, C, Rg = train["X"], train["cls"], train["reg"]
     sur = Surrogate(seed=seed, n_models=n_models)
-    margin = z * math.sqrt(target * (1 - target) / max(nv, 1))
+    nh = len(heldout["cls"])
+    margin = z * math.sqrt(target * (1 - target) * (1.0 / max(nv, 1) + 1.0 / nh))
     dem = None
     try:
         t0 = time.perf_counter()
