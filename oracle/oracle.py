@@ -151,6 +151,26 @@ def ref_available() -> bool:
     return os.path.exists(REF_HARNESS)
 
 
+def run_reference_csv(state: State, dt: float, kin: float, kout: float, n_steps: int, skip: int,
+                      outdir: str) -> None:
+    """The reference's own sim_init_write and sim_looper through ref_harness --csv:
+    constants.csv, simData.csv and energy.csv written into ``outdir``."""
+    if not ref_available():
+        raise FileNotFoundError(REF_HARNESS)
+    os.makedirs(outdir, exist_ok=True)
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<4i", REF_MAGIC, state.n, n_steps, skip))
+            f.write(struct.pack("<3d", dt, kin, kout))
+            for a in (state.pos, state.vel, state.w, state.R, state.m, state.moi):
+                f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+        r = subprocess.run([REF_HARNESS, "--csv", fin, os.path.abspath(outdir)], capture_output=True,
+                           text=True, cwd=td)
+        if r.returncode != 0:
+            raise RuntimeError(f"ref_harness --csv failed ({r.returncode}): {r.stderr[-2000:]}")
+
+
 def run_reference(state: State, dt: float, kin: float, kout: float, n_steps: int, skip: int):
     """Run the reference's own sim_one_step through oracle/_ref/ref_harness.
 

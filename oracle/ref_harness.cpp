@@ -6,6 +6,9 @@
 // linalg.hpp).  Nothing of the reference is copied: the file is included where
 // it lies.  Output goes to oracle/_ref/ only.
 //
+// Usage: ref_harness <input.bin> <output.bin>          state and energies
+//        ref_harness --csv <input.bin> <output-dir>    the reference's own CSV files
+//
 // Protocol (little-endian binary files):
 //   input : int32 {magic 0x534C4731, n, n_steps, skip}
 //           f64   {dt, kin, kout}
@@ -24,6 +27,13 @@
 //      avoids reaching into the class's private section.
 //   3. for Step = 1 .. n_steps-1: sim_one_step(Step % skip == 0); energies of
 //      write steps are recorded and reset as sim_looper does (:333-344).
+//
+// CSV mode: after the step-0 pass, O.sim_init_write(<output-dir>/) writes
+// constants.csv and the headers and first rows of simData.csv and energy.csv
+// (ball_group.hpp:421-516), then the reference's own sim_looper
+// (ColliderSingleCore.cpp:291-384) runs Step = 1 .. n_steps-1 with its write,
+// energy-row and flush logic (precision 12 on its energy stream, :42).  Rows still
+// buffered when sim_looper returns are not written, as in the reference.
 //
 // The reference's translation unit constructs a global Ball_group from two CSV
 // files at a fixed relative path (ColliderSingleCore.cpp:33, dust_const.hpp:65-66)
@@ -116,11 +126,13 @@ void put_energy(std::vector<double>& out)
 
 int main(int argc, char** argv)
 {
-    if (argc != 3) {
-        std::fprintf(stderr, "usage: ref_harness <input.bin> <output.bin>\n");
+    const bool csv = argc == 4 && std::strcmp(argv[1], "--csv") == 0;
+    if (argc != 3 && !csv) {
+        std::fprintf(stderr, "usage: ref_harness <input.bin> <output.bin> | --csv <input.bin> <output-dir>\n");
         cleanup();
         return 2;
     }
+    if (csv) argv++;
     FILE* fi = std::fopen(argv[1], "rb");
     if (!fi) { cleanup(); return 2; }
     int32_t hdr[4];
@@ -162,6 +174,20 @@ int main(int argc, char** argv)
     std::vector<double> e_init, e_w;
     dt = 0;
     sim_one_step(true);
+    if (csv) {
+        // the step-0 pass stands for init_conditions: its row is not the reference's
+        ballBuffer.str("");
+        std::string dir = argv[2];
+        if (dir.empty() || dir.back() != '/') dir += '/';
+        O.sim_init_write(dir);
+        energyBuffer.precision(12);   // as the reference's main does (ColliderSingleCore.cpp:42)
+        steps = n_steps;
+        skip = skip_in;
+        dt = prm[0];
+        sim_looper();
+        cleanup();
+        return 0;
+    }
     put_energy(e_init);
     ballBuffer.str("");
     dt = prm[0];

@@ -159,6 +159,17 @@ def lib():
                                    C.c_double, C.c_double, C.c_double] + [dp] * 7
     L.sl_collision_outcome.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, C.c_double, C.POINTER(Outcome)]
     L.sl_collision_approach.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, dp]
+    cp, ll = C.c_char_p, C.c_longlong
+    for name, args in (("sl_csv_sim_header", [C.c_int, cp, ll]),
+                       ("sl_csv_sim_row", [C.c_int, dp, dp, dp, cp, ll]),
+                       ("sl_csv_energy_row", [C.c_double, C.c_double, C.c_double, dp, dp, C.c_int, cp, ll]),
+                       ("sl_csv_constants", [C.c_int, dp, dp, dp, cp, ll]),
+                       ("sl_csv_last_line", [cp, cp, ll])):
+        getattr(L, name).argtypes = args
+        getattr(L, name).restype = ll
+    L.sl_csv_count_balls.argtypes = [cp]
+    L.sl_csv_parse_sim_row.argtypes = [cp, C.c_int, dp, dp, dp]
+    L.sl_csv_parse_constants.argtypes = [cp, C.c_int, dp, dp, dp]
     L.sl_collision_descriptors.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_double,
                                            C.c_double, dp]
     fp = C.POINTER(C.c_float)
@@ -644,6 +655,70 @@ class MLP:
         self._check(lib().sl_mlp_train_step(self._m, x.shape[0], _fp(x), _ip(cls), _fp(reg), lr, C.byref(loss)),
                     "sl_mlp_train_step")
         return float(loss.value)
+
+
+def _csv_text(fn, what: str, *args) -> str:
+    n = fn(*args, None, 0)
+    if n < 0:
+        raise SpacelabError(int(n), what)
+    buf = C.create_string_buffer(int(n) + 1)
+    fn(*args, buf, n + 1)
+    return buf.raw[:n].decode()
+
+
+def csv_sim_header(n: int) -> str:
+    """simData.csv header line for n balls (sl_csv_sim_header)."""
+    return _csv_text(lib().sl_csv_sim_header, "sl_csv_sim_header", int(n))
+
+
+def csv_sim_row(pos, w, vel) -> str:
+    """One simData.csv row, without its leading newline (sl_csv_sim_row)."""
+    n = len(pos)
+    return _csv_text(lib().sl_csv_sim_row, "sl_csv_sim_row", n, _dp(_f64(pos, (n, 3))), _dp(_f64(w, (n, 3))),
+                     _dp(_f64(vel, (n, 3))))
+
+
+def csv_energy_row(time: float, PE: float, KE: float, p, L, digits: int = 12) -> str:
+    """One energy.csv row "t,PE,KE,E,|p|,|L|", without its leading newline."""
+    return _csv_text(lib().sl_csv_energy_row, "sl_csv_energy_row", float(time), float(PE), float(KE),
+                     _dp(_f64(p, (3,))), _dp(_f64(L, (3,))), int(digits))
+
+
+def csv_constants(R, m, moi) -> str:
+    """The constants.csv text (one "R,m,moi" line per ball)."""
+    n = len(R)
+    return _csv_text(lib().sl_csv_constants, "sl_csv_constants", n, _dp(_f64(R, (n,))), _dp(_f64(m, (n,))),
+                     _dp(_f64(moi, (n,))))
+
+
+def csv_last_line(path: str) -> str:
+    """The last line of a file as the reference's getLastLine reads it."""
+    return _csv_text(lib().sl_csv_last_line, f"sl_csv_last_line({path})", os.fsencode(path))
+
+
+def csv_parse_sim_row(row: str):
+    """(pos, w, vel) of every ball of a simData.csv row."""
+    b = row.encode()
+    n = lib().sl_csv_count_balls(b)
+    pos, w, vel = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+    rc = lib().sl_csv_parse_sim_row(b, n, _dp(pos), _dp(w), _dp(vel))
+    if rc:
+        raise SpacelabError(rc, "sl_csv_parse_sim_row")
+    return pos, w, vel
+
+
+def csv_parse_constants(text: str, n: int):
+    """(R, m, moi) of n balls from constants.csv text."""
+    R, m, moi = np.zeros(n), np.zeros(n), np.zeros(n)
+    rc = lib().sl_csv_parse_constants(text.encode(), n, _dp(R), _dp(m), _dp(moi))
+    if rc:
+        raise SpacelabError(rc, "sl_csv_parse_constants")
+    return R, m, moi
+
+
+def collider_program() -> str:
+    """Path of the C++ driver (spacelab_collider) built beside the library."""
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "spacelab_collider")
 
 
 def probe_rotation() -> np.ndarray:

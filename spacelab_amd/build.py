@@ -22,7 +22,10 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libspacelab.so")
 
 SOURCES = ["sl_api.hip", "sl_kernels_ordered.hip", "sl_kernels_fast.hip", "sl_util.hip",
-           "sl_batch.hip", "sl_mlp.hip", "sl_host.cpp", "sl_collide.cpp"]
+           "sl_batch.hip", "sl_mlp.hip", "sl_host.cpp", "sl_collide.cpp", "sl_csv.cpp"]
+HOSTDIR = os.path.join(PKG, "host")
+# host programs over the C ABI: (source, executable in lib/)
+PROGRAMS = [("collider_main.cpp", "spacelab_collider")]
 HEADERS = ["sl_common.hpp", "sl_pair_ref.hpp", "sl_internal.hpp", "sl_fast_pair.hpp",
            "sl_fast_kernels.hpp"]
 
@@ -67,6 +70,15 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    for src, exe in PROGRAMS:
+        out = os.path.join(LIBDIR, exe)
+        cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+               "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + HOSTDIR, os.path.join(HOSTDIR, src),
+               "-L" + LIBDIR, "-lspacelab", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host program build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(out + ".tmp", out)
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -1,0 +1,373 @@
+// C++ host driver over the C ABI: a Ball_group with the reference's arrays and
+// loaders, and a Collider that runs sim_one_step / sim_looper on the GPU and writes
+// the run's constants, simData and energy files in the reference's layout.
+//
+// Reference counterparts: Ball_group (ball_group.hpp:21-53 arrays, :1236-1247 and
+// :1352-1364 loadSim/sim_continue via getLastLine/parseSimData/loadConsts, :421-516
+// sim_init_write, :519-547 getCOM/to_origin, :1117-1124 getMass), calibrate_dt's first call
+// (:197-238: updateDTK at a custom speed, then steps and skip from the simulated time
+// and the time resolution) and the driver's sim_one_step / sim_looper
+// (ColliderSingleCore.cpp:78-288, :291-384).
+//
+// The state lives on the GPU between steps; it is downloaded on write steps only
+// (the simData row needs it).  Non-write steps between two write steps go to the
+// library in one call, which gives the same results as one call per step.
+#pragma once
+
+#include "spacelab/sl_api.h"
+#include "spacelab/sl_csv.h"
+
+#include <cmath>
+#include <cstdio>
+#include <ctime>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace spacelab {
+
+struct vec3 {
+    double x = 0, y = 0, z = 0;
+    double norm() const { return std::sqrt(x * x + y * y + z * z); }
+    vec3& operator+=(const vec3& o)
+    {
+        x += o.x;
+        y += o.y;
+        z += o.z;
+        return *this;
+    }
+};
+static_assert(sizeof(vec3) == 3 * sizeof(double), "vec3 must be three packed doubles (the ABI's AoS layout)");
+
+// Constants of the reference's run control (dust_const.hpp:28-29).
+constexpr double kSimTimeSeconds = 0.5e-3;
+constexpr double kTimeResolution = 1e-5;
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string& what) : std::runtime_error(what + " (code " + std::to_string(code) + ")"), code(code) {}
+    int code;
+};
+
+inline std::string read_file(const std::string& path)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw Error(SL_E_ARG, "cannot open " + path);
+    std::ostringstream s;
+    s << f.rdbuf();
+    return s.str();
+}
+
+inline bool file_exists(const std::string& path) { return std::ifstream(path).good(); }
+
+class Ball_group {
+public:
+    int num_particles = 0;
+    std::vector<vec3> pos, vel, w, acc, aacc;
+    std::vector<double> R, m, moi;
+    double PE = 0, KE = 0;
+    vec3 mom, ang_mom;
+
+    Ball_group() = default;
+    explicit Ball_group(int n) { allocate(n); }
+
+    void allocate(int n)
+    {
+        num_particles = n;
+        pos.assign(n, vec3{});
+        vel.assign(n, vec3{});
+        w.assign(n, vec3{});
+        acc.assign(n, vec3{});
+        aacc.assign(n, vec3{});
+        R.assign(n, 0.0);
+        m.assign(n, 0.0);
+        moi.assign(n, 0.0);
+    }
+
+    // Continue a run: the last simData row and the constants file of <path><prefix>.
+    static Ball_group from_csv(const std::string& path, const std::string& prefix)
+    {
+        const std::string sim = path + prefix + "simData.csv";
+        const long long len = sl_csv_last_line(sim.c_str(), nullptr, 0);
+        if (len < 0) throw Error((int)len, "cannot read " + sim);
+        std::string line((size_t)len + 1, '\0');
+        sl_csv_last_line(sim.c_str(), &line[0], len + 1);
+        line.resize((size_t)len);
+        Ball_group g(sl_csv_count_balls(line.c_str()));
+        int rc = sl_csv_parse_sim_row(line.c_str(), g.num_particles, &g.pos[0].x, &g.w[0].x, &g.vel[0].x);
+        if (rc) throw Error(rc, "bad simData row in " + sim);
+        const std::string consts = read_file(path + prefix + "constants.csv");
+        rc = sl_csv_parse_constants(consts.c_str(), g.num_particles, g.R.data(), g.m.data(), g.moi.data());
+        if (rc) throw Error(rc, "bad constants file " + path + prefix + "constants.csv");
+        return g;
+    }
+
+    // Seeded ballistic particle-cluster aggregate (sl_gen_bpca).
+    static Ball_group generate(int n, unsigned long long seed, double r_med = 1e-5, double sigma = 0.2,
+                               double rho = 2.25, double v_sigma = 1e-3)
+    {
+        Ball_group g(n);
+        const int rc = sl_gen_bpca(n, seed, r_med, sigma, rho, v_sigma, &g.pos[0].x, &g.vel[0].x, &g.w[0].x,
+                                   g.R.data(), g.m.data(), g.moi.data());
+        if (rc) throw Error(rc, "sl_gen_bpca");
+        return g;
+    }
+
+    double getMass() const
+    {
+        double t = 0;
+        for (int i = 0; i < num_particles; i++) t += m[i];
+        return t;
+    }
+
+    vec3 getCOM() const
+    {
+        const double M = getMass();
+        vec3 c;
+        for (int i = 0; i < num_particles; i++) {
+            c.x += pos[i].x * m[i];
+            c.y += pos[i].y * m[i];
+            c.z += pos[i].z * m[i];
+        }
+        c.x /= M;
+        c.y /= M;
+        c.z /= M;
+        return c;
+    }
+
+    void to_origin()
+    {
+        const vec3 c = getCOM();
+        for (auto& p : pos) {
+            p.x -= c.x;
+            p.y -= c.y;
+            p.z -= c.z;
+        }
+    }
+
+    void zeroVel()
+    {
+        for (auto& v : vel) v = vec3{};
+    }
+    void zeroAngVel()
+    {
+        for (auto& v : w) v = vec3{};
+    }
+};
+
+// kin, kout and dt by updateDTK at a speed (sl_calibrate's reference rule), and the
+// step count and write interval of calibrate_dt's first call.
+struct RunControl {
+    double kin = 0, kout = 0, dt = 0;
+    int steps = 0, skip = 0;
+
+    static RunControl calibrate(const Ball_group& g, double speed, double cor, int dt_rule = 0)
+    {
+        sl_params p;
+        sl_params_preset(&p, 0);
+        const int rc = sl_calibrate(&p, g.num_particles, g.R.data(), g.m.data(), speed, cor, dt_rule);
+        if (rc) throw Error(rc, "sl_calibrate");
+        RunControl r;
+        r.kin = p.kin;
+        r.kout = p.kout;
+        r.dt = p.dt;
+        r.steps = (int)(kSimTimeSeconds / r.dt);
+        r.skip = kTimeResolution / r.dt > 1. ? (int)std::floor(kTimeResolution / r.dt) : (int)std::floor(1. / r.dt);
+        return r;
+    }
+};
+
+class Collider {
+public:
+    std::string output_prefix;
+    double simTimeElapsed = 0;
+    bool progress = true;
+
+    Collider(Ball_group& g, const sl_params& p) : O(g)
+    {
+        int rc = sl_create(&ctx, g.num_particles, &p);
+        if (rc) throw Error(rc, rc == SL_E_NOGPU ? "no usable gfx950 GPU" : "sl_create");
+        rc = sl_upload(ctx, g.num_particles, &g.pos[0].x, &g.vel[0].x, &g.w[0].x, &g.acc[0].x, &g.aacc[0].x,
+                       g.R.data(), g.m.data(), g.moi.data(), nullptr);
+        check(rc, "sl_upload");
+        dt = p.dt;
+    }
+    ~Collider()
+    {
+        if (ctx) sl_destroy(ctx);
+    }
+    Collider(const Collider&) = delete;
+    Collider& operator=(const Collider&) = delete;
+
+    // init_conditions (ball_group.hpp:777-947): forces of the current state and the
+    // energies PE, KE, momentum and angular momentum, into O.
+    void init_conditions()
+    {
+        sl_energy e;
+        check(sl_init_forces(ctx, &e), "sl_init_forces");
+        add_energy(e);
+        download();
+    }
+
+    // sim_init_write (ball_group.hpp:421-516): picks a free file name (a counter
+    // prefix while <prefix>simData.csv exists), writes the constants, both headers,
+    // the current energy row at 6 digits and the current state row, then resets the
+    // energy sums.
+    void sim_init_write(std::string filename, int counter = 0)
+    {
+        bool taken = file_exists(filename + "simData.csv");
+        while (taken) {
+            counter++;
+            taken = file_exists(std::to_string(counter) + '_' + filename + "simData.csv");
+        }
+        if (counter > 0) filename.insert(0, std::to_string(counter) + '_');
+        output_prefix = filename;
+        const int n = O.num_particles;
+        std::string e = SL_CSV_ENERGY_HEADER;
+        e += '\n';
+        e += energy_row(SL_CSV_STATE_DIGITS);
+        append(filename + "energy.csv", e);
+        std::string s = sim_header(n);
+        s += '\n';
+        s += sim_row();
+        append(filename + "simData.csv", s);
+        append(filename + "constants.csv", constants());
+        reset_energy();
+    }
+
+    // sim_one_step (ColliderSingleCore.cpp:78-288): one velocity-Verlet step; on a
+    // write step the state row goes to the simData buffer and the energies are added
+    // to O's sums.
+    void sim_one_step(bool write_step) { steps_then_write(write_step ? 0 : 1, write_step); }
+
+    // sim_looper (ColliderSingleCore.cpp:291-384): Step = 1 .. steps-1, write steps
+    // every `skip`, an energy row at 12 digits after each, the buffers flushed to the
+    // files at the first write and at every tenth (or after 30 minutes).
+    void sim_looper(int steps, int skip)
+    {
+        if (skip < 1) throw Error(SL_E_ARG, "skip must be at least 1");
+        const time_t t0 = std::time(nullptr);
+        int pending = 0;   // non-write steps not yet run
+        for (int Step = 1; Step < steps; Step++) {
+            if (Step % skip != 0) {
+                pending++;
+                continue;
+            }
+            simTimeElapsed += dt * skip;
+            if (progress)
+                std::fprintf(stderr, "%d\t%2.0f%%\tReal: %.1f s\tSim: %.4g s\n", Step,
+                             100.0 * Step / steps, difftime(std::time(nullptr), t0), simTimeElapsed);
+            steps_then_write(pending, true);
+            pending = 0;
+            energyBuffer += '\n';
+            energyBuffer += energy_row(SL_CSV_ENERGY_DIGITS);
+            reset_energy();
+            if (std::time(nullptr) - lastWrite > 1800 || Step / skip % 10 == 0) flush();
+        }
+        if (pending) steps_then_write(pending, false);
+    }
+
+    // Appends the buffered rows to the files.  The reference flushes only inside the
+    // loop, so rows buffered after its last flush are not written by it.
+    void flush()
+    {
+        append(output_prefix + "simData.csv", ballBuffer);
+        append(output_prefix + "energy.csv", energyBuffer);
+        ballBuffer.clear();
+        energyBuffer.clear();
+        lastWrite = std::time(nullptr);
+    }
+
+    void download()
+    {
+        check(sl_download(ctx, &O.pos[0].x, &O.vel[0].x, &O.w[0].x, &O.acc[0].x, &O.aacc[0].x), "sl_download");
+    }
+
+    sl_ctx* context() { return ctx; }
+
+private:
+    Ball_group& O;
+    sl_ctx* ctx = nullptr;
+    double dt = 0;
+    std::string ballBuffer, energyBuffer;
+    time_t lastWrite = 0;
+
+    void check(int rc, const char* what)
+    {
+        if (rc < 0) throw Error(rc, std::string(what) + ": " + sl_last_error(ctx));
+    }
+
+    // `n_plain` non-write steps, then one step that is a write step if `write`
+    void steps_then_write(int n_plain, bool write)
+    {
+        if (n_plain > 0) check(sl_step(ctx, n_plain, 0, nullptr, 0), "sl_step");
+        if (!write) return;
+        sl_energy e;
+        check(sl_step(ctx, 1, 1, &e, 1), "sl_step");
+        add_energy(e);
+        download();
+        ballBuffer += '\n';
+        ballBuffer += sim_row();
+    }
+
+    void add_energy(const sl_energy& e)
+    {
+        O.PE += e.PE;
+        O.KE += e.KE;
+        O.mom += vec3{e.p[0], e.p[1], e.p[2]};
+        O.ang_mom += vec3{e.L[0], e.L[1], e.L[2]};
+    }
+
+    void reset_energy()
+    {
+        O.PE = 0;
+        O.KE = 0;
+        O.mom = vec3{};
+        O.ang_mom = vec3{};
+    }
+
+    template <class F>
+    static std::string text(F f)
+    {
+        const long long len = f(nullptr, 0);
+        if (len < 0) throw Error((int)len, "CSV formatting");
+        std::string s((size_t)len + 1, '\0');
+        f(&s[0], len + 1);
+        s.resize((size_t)len);
+        return s;
+    }
+
+    std::string sim_header(int n)
+    {
+        return text([&](char* b, long long c) { return sl_csv_sim_header(n, b, c); });
+    }
+    std::string sim_row()
+    {
+        return text([&](char* b, long long c) {
+            return sl_csv_sim_row(O.num_particles, &O.pos[0].x, &O.w[0].x, &O.vel[0].x, b, c);
+        });
+    }
+    std::string energy_row(int digits)
+    {
+        return text([&](char* b, long long c) {
+            return sl_csv_energy_row(simTimeElapsed, O.PE, O.KE, &O.mom.x, &O.ang_mom.x, digits, b, c);
+        });
+    }
+    std::string constants()
+    {
+        return text([&](char* b, long long c) {
+            return sl_csv_constants(O.num_particles, O.R.data(), O.m.data(), O.moi.data(), b, c);
+        });
+    }
+
+    static void append(const std::string& path, const std::string& s)
+    {
+        std::ofstream f(path, std::ios::binary | std::ios::app);
+        if (!f) throw Error(SL_E_ARG, "cannot write " + path);
+        f << s;
+    }
+};
+
+}  // namespace spacelab

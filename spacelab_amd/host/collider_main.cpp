@@ -1,0 +1,121 @@
+// spacelab_collider: a ColliderSingleCore-shaped driver on the GPU.
+//
+// Builds the Ball_group (continued from a run's CSV files, as
+// Ball_group(path, filename, customVel) does, ball_group.hpp:92-98, or generated),
+// centres it on its centre of mass, calibrates kin, kout and dt at a custom speed
+// (calibrate_dt -> updateDTK, ball_group.hpp:197-238,1313-1329) unless they are given,
+// runs init_conditions, sim_init_write and sim_looper (ColliderSingleCore.cpp:291-384)
+// and writes <out>constants.csv, <out>simData.csv and <out>energy.csv.
+//
+//   spacelab_collider (--continue PATH PREFIX | --generate N SEED) [--out PREFIX]
+//       [--v-custom V] [--cor C] [--dt DT --kin KIN --kout KOUT] [--steps N] [--skip K]
+//       [--order ordered|fast] [--precision fp64|mixed] [--preset reference|config]
+//       [--no-center] [--quiet]
+#include "collider.hpp"
+
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+
+namespace {
+
+[[noreturn]] void usage(const char* msg)
+{
+    std::fprintf(stderr, "spacelab_collider: %s\n"
+                         "usage: spacelab_collider (--continue PATH PREFIX | --generate N SEED) [--out PREFIX]\n"
+                         "  [--v-custom V] [--cor C] [--dt DT --kin KIN --kout KOUT] [--steps N] [--skip K]\n"
+                         "  [--order ordered|fast] [--precision fp64|mixed] [--preset reference|config]\n"
+                         "  [--no-center] [--quiet]\n",
+                 msg);
+    std::exit(2);
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    using namespace spacelab;
+    std::string path, prefix, out = "Unnamed_";   // output_prefix default, dust_const.hpp:67
+    int gen_n = 0;
+    unsigned long long gen_seed = 0;
+    double v_custom = 0.36301555459799423;        // dust_const.hpp:34
+    double cor = 0.4;                             // dust_const.hpp:27
+    double dt = 0, kin = 0, kout = 0;
+    int steps = -1, skip = -1, order = SL_ORDER_ORDERED, precision = SL_PREC_FP64, preset = 0;
+    bool center = true, quiet = false, cont = false;
+    for (int i = 1; i < argc; i++) {
+        const std::string a = argv[i];
+        auto next = [&]() -> const char* {
+            if (i + 1 >= argc) usage(("missing value after " + a).c_str());
+            return argv[++i];
+        };
+        if (a == "--continue") {
+            path = next();
+            prefix = next();
+            cont = true;
+        } else if (a == "--generate") {
+            gen_n = std::atoi(next());
+            gen_seed = std::strtoull(next(), nullptr, 0);
+        } else if (a == "--out") out = next();
+        else if (a == "--v-custom") v_custom = std::atof(next());
+        else if (a == "--cor") cor = std::atof(next());
+        else if (a == "--dt") dt = std::atof(next());
+        else if (a == "--kin") kin = std::atof(next());
+        else if (a == "--kout") kout = std::atof(next());
+        else if (a == "--steps") steps = std::atoi(next());
+        else if (a == "--skip") skip = std::atoi(next());
+        else if (a == "--order") {
+            const std::string v = next();
+            order = v == "fast" ? SL_ORDER_FAST : v == "ordered" ? SL_ORDER_ORDERED : -1;
+            if (order < 0) usage("--order is ordered or fast");
+        } else if (a == "--precision") {
+            const std::string v = next();
+            precision = v == "mixed" ? SL_PREC_MIXED : v == "fp64" ? SL_PREC_FP64 : -1;
+            if (precision < 0) usage("--precision is fp64 or mixed");
+        } else if (a == "--preset") {
+            const std::string v = next();
+            preset = v == "config" ? 1 : v == "reference" ? 0 : -1;
+            if (preset < 0) usage("--preset is reference or config");
+        } else if (a == "--no-center") center = false;
+        else if (a == "--quiet") quiet = true;
+        else usage(("unknown argument " + a).c_str());
+    }
+    if (cont == (gen_n > 0)) usage("give exactly one of --continue and --generate");
+    try {
+        Ball_group O = cont ? Ball_group::from_csv(path, prefix) : Ball_group::generate(gen_n, gen_seed);
+        if (center) O.to_origin();
+        RunControl rc = RunControl::calibrate(O, v_custom, cor);
+        if (dt > 0) rc.dt = dt;
+        if (kin > 0) rc.kin = kin;
+        if (kout > 0) rc.kout = kout;
+        if (steps >= 0) rc.steps = steps;
+        if (skip > 0) rc.skip = skip;
+        sl_params p;
+        sl_params_preset(&p, preset);
+        p.kin = rc.kin;
+        p.kout = rc.kout;
+        p.dt = rc.dt;
+        p.order = order;
+        p.precision = precision;
+        if (!quiet)
+            std::cerr << "==================\ndt: " << p.dt << "\nk: " << p.kin << "\nSkip: " << rc.skip
+                      << "\nSteps: " << rc.steps << "\n==================\n";
+        Collider c(O, p);
+        c.progress = !quiet;
+        c.init_conditions();
+        c.sim_init_write(out);
+        const time_t t0 = std::time(nullptr);
+        c.sim_looper(rc.steps, rc.skip);
+        c.flush();
+        if (!quiet)
+            std::cerr << "Simulation complete!\n"
+                      << O.num_particles << " Particles and " << rc.steps << " Steps.\n"
+                      << "Simulated time: " << rc.steps * p.dt << " seconds\n"
+                      << "Computation time: " << std::time(nullptr) - t0 << " seconds\n"
+                      << "Output: " << c.output_prefix << "{constants,simData,energy}.csv\n";
+    } catch (const Error& e) {
+        std::fprintf(stderr, "spacelab_collider: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

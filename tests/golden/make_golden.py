@@ -9,6 +9,10 @@ read.
 
     python tests/golden/make_golden.py
 
+It also keeps the reference's own CSV files of a short run (``csv_cluster16/``:
+the input files a continued run starts from and the files the reference's
+sim_init_write and sim_looper write), for the CSV drop-in tests.
+
 Cases follow the golden recipe for this path: two spheres head-on (loading then
 unloading), two spheres oblique with spin (sliding and rolling friction), three
 spheres of unequal radii, and 16- and 64-sphere touching clusters; 1, 10 and 100
@@ -128,11 +132,51 @@ CASES = {
 }
 
 
+def round6(a):
+    """Values as a continued run reads them back from the CSV files: printed at 6
+    significant digits and parsed again."""
+    a = np.asarray(a, dtype=np.float64)
+    return np.array([float("%.6g" % x) for x in a.ravel()]).reshape(a.shape)
+
+
+CSV_CASE = "csv_cluster16"
+CSV_RUN = (61, 3)   # steps, skip: write steps 3..60, the last one a flush point
+
+
+def make_csv_case():
+    """The reference's own files of a short run (sim_init_write + sim_looper through
+    ref_harness --csv), from the 16-sphere cluster rounded through the CSV text.  The
+    input files of the run are the reference's own files of a zero-step run from the
+    same state."""
+    pos, vel, w, R, m, kin, kout, dt = case_cluster(16, 1601)
+    mo = np.array([moi(r, mm) for r, mm in zip(R, m)])
+    pos, vel, w, R, m, mo = (round6(a) for a in (pos, vel, w, R, m, mo))
+    st = oracle.State(pos, vel, w, R, m, mo)
+    d = os.path.join(OUT, CSV_CASE)
+    for sub in ("input", "expected"):
+        sd = os.path.join(d, sub)
+        os.makedirs(sd, exist_ok=True)
+        for f in os.listdir(sd):
+            os.remove(os.path.join(sd, f))
+    oracle.run_reference_csv(st, dt, kin, kout, 1, 1, os.path.join(d, "input"))
+    os.remove(os.path.join(d, "input", "energy.csv"))
+    n_steps, skip = CSV_RUN
+    oracle.run_reference_csv(st, dt, kin, kout, n_steps, skip, os.path.join(d, "expected"))
+    out, e_init, ew = oracle.run_reference(st, dt, kin, kout, n_steps, skip)
+    fname = os.path.join(OUT, f"{CSV_CASE}_s{n_steps - 1}_k{skip}.npz")
+    np.savez(fname, pos=st.pos, vel=st.vel, w=st.w, R=st.R, m=st.m, moi=st.moi,
+             params=np.array([dt, kin, kout]), n_steps=np.array(n_steps), skip=np.array(skip),
+             out_pos=out["pos"], out_vel=out["vel"], out_velh=out["velh"], out_w=out["w"],
+             out_wh=out["wh"], out_acc=out["acc"], out_aacc=out["aacc"], e_init=e_init, e_writes=ew)
+    print(f"{CSV_CASE}: reference CSV files and {os.path.basename(fname)}")
+
+
 def main():
     if not oracle.ref_available():
         oracle.build()
     if not oracle.ref_available():
         raise SystemExit("reference harness not built (reference sources absent)")
+    make_csv_case()
     for name, (fn, runs) in CASES.items():
         pos, vel, w, R, m, kin, kout, dt = fn()
         mo = np.array([moi(r, mm) for r, mm in zip(R, m)])

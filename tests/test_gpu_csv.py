@@ -1,0 +1,90 @@
+"""The C++ driver (spacelab_collider) end to end on the GPU: it continues a run from
+the reference's input files and writes constants.csv and simData.csv byte for byte
+as the reference's own sim_init_write + sim_looper do on the same input (the ordered
+path is bitwise the reference's arithmetic).  energy.csv has the same layout and
+times; its sums are taken in another order than the reference's running sums, so
+PE, KE, E and |L| agree to 1e-9 relative and |p| (a sum that cancels to rounding
+noise) to the momentum scale."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, load_golden
+
+import spacelab_amd as sl
+
+pytestmark = pytest.mark.gpu
+
+CASE = os.path.join(GOLDEN, "csv_cluster16")
+
+
+def _run(tmp, extra, cwd=None, out=None):
+    g = load_golden("csv_cluster16_s60_k3")
+    for f in ("constants.csv", "simData.csv"):
+        shutil.copy(os.path.join(CASE, "input", f), os.path.join(tmp, "in_" + f))
+    dt, kin, kout = (repr(float(x)) for x in g["params"])
+    cmd = [sl.collider_program(), "--continue", str(tmp) + "/", "in_",
+           "--out", out if out is not None else str(tmp) + "/out_",
+           "--no-center", "--dt", dt, "--kin", kin, "--kout", kout,
+           "--steps", str(int(g["n_steps"])), "--skip", str(int(g["skip"])),
+           "--preset", "reference", "--quiet"] + extra
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120, cwd=cwd)
+    assert r.returncode == 0, r.stderr
+    return g
+
+
+def _energy(path):
+    lines = open(path).read().split("\n")
+    return lines[0], [l.split(",") for l in lines[1:]]
+
+
+def test_driver_matches_reference_files(tmp_path):
+    g = _run(tmp_path, ["--order", "ordered"])
+    for f in ("constants.csv", "simData.csv"):
+        got = open(tmp_path / ("out_" + f), "rb").read()
+        exp = open(os.path.join(CASE, "expected", f), "rb").read()
+        assert got == exp, f
+    h1, a = _energy(tmp_path / "out_energy.csv")
+    h2, b = _energy(os.path.join(CASE, "expected", "energy.csv"))
+    assert h1 == h2 and len(a) == len(b) == 21
+    pscale = float(np.sum(g["m"] * np.linalg.norm(g["vel"], axis=1)))
+    for ra, rb in zip(a, b):
+        assert ra[0] == rb[0]                                   # time column, same text
+        x, y = np.array(ra[1:], float), np.array(rb[1:], float)
+        esc = abs(y[0]) + abs(y[1])
+        assert abs(x[0] - y[0]) <= 1e-9 * esc and abs(x[1] - y[1]) <= 1e-9 * esc   # PE, KE
+        assert abs(x[2] - y[2]) <= 1e-9 * esc                   # E
+        assert abs(x[3] - y[3]) <= 1e-12 * pscale               # |p|
+        assert abs(x[4] - y[4]) <= 1e-9 * abs(y[4])             # |L|
+
+
+def test_driver_fast_mode_close(tmp_path):
+    _run(tmp_path, ["--order", "fast"])
+    got = open(tmp_path / "out_simData.csv").read().split("\n")
+    exp = open(os.path.join(CASE, "expected", "simData.csv")).read().split("\n")
+    assert got[0] == exp[0] and len(got) == len(exp)
+    for la, lb in zip(got[1:], exp[1:]):
+        pa, wa, va = sl.csv_parse_sim_row(la)
+        pb, wb, vb = sl.csv_parse_sim_row(lb)
+        # the fast path reorders sums; printed at 6 digits the states agree to that
+        assert np.allclose(pa, pb, rtol=2e-5, atol=1e-5 * np.abs(pb).max())
+        assert np.allclose(va, vb, rtol=2e-5, atol=1e-5 * np.abs(vb).max())
+
+
+def test_driver_file_counter(tmp_path):
+    """A second run with the same output prefix gets a "1_" prefix, as sim_init_write does."""
+    _run(tmp_path, ["--order", "ordered"], cwd=tmp_path, out="out_")
+    _run(tmp_path, ["--order", "ordered"], cwd=tmp_path, out="out_")
+    a = open(tmp_path / "out_simData.csv", "rb").read()
+    b = open(tmp_path / "1_out_simData.csv", "rb").read()
+    assert a == b
+
+
+def test_driver_rejects_bad_input(tmp_path):
+    r = subprocess.run([sl.collider_program(), "--continue", str(tmp_path) + "/", "nothing_"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "cannot" in r.stderr
