@@ -168,12 +168,15 @@ void sl_batch_destroy(sl_batch* b);
 
 /* ---- measurement ---------------------------------------------------------- */
 typedef struct {
-    double pair_ms;        /* summed duration of the pair kernel launches (HIP events) */
+    double pair_ms;        /* summed duration of the pair kernel launches (HIP events); in
+                              fast mode the tile sweep (far-field pairs) only */
     long long pair_launches;
     double ball_ms;        /* summed duration of the per-ball kernels */
     long long ball_launches;
     long long contacts;    /* overlapping pairs summed over profiled steps */
     long long near_pairs;  /* pairs routed to the near/contact path (fast mode) */
+    double near_ms;        /* summed duration of the near-pair kernel launches (fast mode) */
+    long long near_launches;
 } sl_profile;
 
 /* Record HIP events around each kernel launch of sl_step (on the context's

@@ -50,7 +50,7 @@ FastLayout fast_layout(int n_sys, const int* n_per_sys);
 
 // Optional hooks called around each kernel launch (profiling with HIP events).
 struct LaunchHook {
-    void (*before)(void* user, int kind) = nullptr;   // kind 0: pair/tile kernel, 1: per-ball kernel
+    void (*before)(void* user, int kind) = nullptr;   // kind 0: pair/tile sweep, 1: per-ball, 2: near pairs
     void (*after)(void* user, int kind) = nullptr;
     void* user = nullptr;
 };

@@ -226,8 +226,10 @@ int launch_tiles_step(FastBuffers& fb, bool write, hipStream_t st, std::string* 
     }
     if (fb.hook.before) fb.hook.before(fb.hook.user, 0);
     launch_far_any(a, fb.split, fb.precision, write, gravity_on(fb), st);
-    launch_near(a, write, st);
     if (fb.hook.after) fb.hook.after(fb.hook.user, 0);
+    if (fb.hook.before) fb.hook.before(fb.hook.user, 2);
+    launch_near(a, write, st);
+    if (fb.hook.after) fb.hook.after(fb.hook.user, 2);
     if (hipGetLastError() != hipSuccess) return err_hip(err, "tile kernel launch");
     return 0;
 }
@@ -333,7 +335,11 @@ int graph_segment(FastBuffers& fb, int len, bool write_last, hipStream_t st, std
     const FastBuffers::GraphKey key{len, write_last ? 1 : 0, fb.cur};
     auto it = fb.graphs.find(key);
     if (it == fb.graphs.end()) {
-        if (fb.graphs.size() >= 16) fast_clear_graphs(fb);
+        if (fb.graphs.size() >= 16) {
+            // earlier segments may still be running: destroy their graphs once idle
+            if (hipStreamSynchronize(st) != hipSuccess) return err_hip(err, "synchronize");
+            fast_clear_graphs(fb);
+        }
         const int cur0 = fb.cur;
         hipGraph_t g = nullptr;
         if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
