@@ -286,20 +286,19 @@ def main():
         pr = r["prof"]
         ps = max(r["prof_steps"], 1)
         pair_ms = pr["pair_ms"] / max(pr["pair_launches"], 1)
-        near_ms = pr["near_ms"] / pr["near_launches"] if pr["near_launches"] else 0.0
+        ball_ms = pr["ball_ms"] / max(pr["ball_launches"], 1)
         contacts = pr["contacts"] / ps
         near = pr["near_pairs"] / ps
         f_far = F_FAR[p.far_gravity]
         flops = f_far * (pairs - contacts) + F_CONTACT * contacts    # the whole pair pass
         d = {"pairs_per_s": r["pairs_per_s"], "ms_per_step": r["ms_per_step"],
-             "ball_kernel_ms": pr["ball_ms"] / max(pr["ball_launches"], 1),
-             "contacts_per_step": contacts}
-        if pr["near_launches"]:
-            # fast modes: the tile sweep evaluates the far-field pairs, the near pass the
-            # pairs closer than S + h_min (contacts at 214 flops, the rest at far cost)
-            d.update({"tile_kernel_ms": pair_ms, "near_kernel_ms": near_ms, "near_pairs_per_step": near,
+             "ball_kernel_ms": ball_ms, "contacts_per_step": contacts}
+        if name != "fp64-ordered":
+            # fast modes: the tile sweep evaluates the far-field pairs; the per-ball kernel
+            # the pairs closer than S + h_min (contacts at 214 flops, the rest at far cost)
+            d.update({"tile_kernel_ms": pair_ms, "near_pairs_per_step": near,
                       "tile_kernel_tflops": f_far * (pairs - near) / (pair_ms * 1e-3) / 1e12,
-                      "pair_pass_tflops": flops / ((pair_ms + near_ms) * 1e-3) / 1e12})
+                      "step_kernels_tflops": flops / ((pair_ms + ball_ms) * 1e-3) / 1e12})
         else:
             d.update({"pair_kernel_ms": pair_ms,
                       "pair_kernel_tflops": flops / (pair_ms * 1e-3) / 1e12 if pair_ms > 0 else None})

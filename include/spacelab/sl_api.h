@@ -169,14 +169,13 @@ void sl_batch_destroy(sl_batch* b);
 /* ---- measurement ---------------------------------------------------------- */
 typedef struct {
     double pair_ms;        /* summed duration of the pair kernel launches (HIP events); in
-                              fast mode the tile sweep (far-field pairs) only */
+                              fast mode the far-field tile sweep */
     long long pair_launches;
-    double ball_ms;        /* summed duration of the per-ball kernels */
+    double ball_ms;        /* summed duration of the per-ball kernels; in fast mode they
+                              also evaluate the near pairs */
     long long ball_launches;
     long long contacts;    /* overlapping pairs summed over profiled steps */
     long long near_pairs;  /* pairs routed to the near/contact path (fast mode) */
-    double near_ms;        /* summed duration of the near-pair kernel launches (fast mode) */
-    long long near_launches;
 } sl_profile;
 
 /* Record HIP events around each kernel launch of sl_step (on the context's

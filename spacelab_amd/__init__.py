@@ -78,8 +78,7 @@ N_DESCRIPTORS = 21
 class _Profile(C.Structure):
     _fields_ = [("pair_ms", C.c_double), ("pair_launches", C.c_longlong),
                 ("ball_ms", C.c_double), ("ball_launches", C.c_longlong),
-                ("contacts", C.c_longlong), ("near_pairs", C.c_longlong),
-                ("near_ms", C.c_double), ("near_launches", C.c_longlong)]
+                ("contacts", C.c_longlong), ("near_pairs", C.c_longlong)]
 
 
 @dataclass

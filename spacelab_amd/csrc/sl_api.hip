@@ -149,9 +149,6 @@ int flush_profile(sl_ctx* c)
         if (mk.kind == 0) {
             c->acc_prof.pair_ms += ms;
             c->acc_prof.pair_launches++;
-        } else if (mk.kind == 2) {
-            c->acc_prof.near_ms += ms;
-            c->acc_prof.near_launches++;
         } else {
             c->acc_prof.ball_ms += ms;
             c->acc_prof.ball_launches++;

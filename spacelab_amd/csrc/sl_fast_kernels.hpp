@@ -107,12 +107,7 @@ struct TileArgs {
     double Kf;              // mixed: force = Kf * (scaled binary32 factor) * (r / L0)
     double* part;           // [slot][xyz][n_pad]
     unsigned long long* nbm;   // [slot][n_pad]: near-partner masks
-    double* nd;             // [slot][F xyz, T xyz][n_pad]: near sums, where the mask is nonzero
-    double* pe_tile;        // per tile: pair energy (write steps)
-    int* tile_nc;           // per tile: touching pairs (write steps)
-    unsigned long long* tile_ctr;   // per tile: running sums of touching and near pairs
-    State cur;              // full state, read by the near-pair evaluation
-    const int* perm;        // engine slot -> the caller's ball index (fixes pair orientation)
+    double* pe_tile;        // per tile: far-field pair energy (write steps)
 };
 
 // Sum over the 64 lanes of a wavefront as a fixed tree (deterministic).
@@ -205,7 +200,7 @@ struct JRec64 {
 // The pair sweep of every tile: the far-field force of each pair, with the per-pair
 // near test only in tiles the bounding-box test cannot rule out.  A near pair
 // (closer than S + h_min) contributes nothing here; it is recorded as a bit of the
-// partner masks of both balls, for k_near.  SL_PREC_MIXED evaluates the factor in
+// partner masks of both balls, for k_ball.  SL_PREC_MIXED evaluates the factor in
 // binary32 from binary64 geometry, in units of L0 (median radius) and M0 (median
 // mass), as one sum for vdW and gravity: f = Kf (q_i q_j / (s t)^2 + g_i g_j / d^3)
 // with per-ball weights q = sqrt(Kv/Kf) (R/L0)^3, g = sqrt(Kg/Kf) m/M0, accumulated in
@@ -467,11 +462,6 @@ __device__ __forceinline__ D3 load3(const State& c, int f, int b)
     return d3(c.f[f][b], c.f[f + 1][b], c.f[f + 2][b]);
 }
 
-struct NearAcc {
-    double F[3], T[3], pe;
-    int contacts, pairs;
-};
-
 // Result of one (ball, partner) near evaluation, for the ball: force, torque, and the
 // pair's energy and touching flag when the ball is A (counted once per pair).
 struct NearRes {
@@ -479,8 +469,6 @@ struct NearRes {
     double pe;
     int contact, is_a;
 };
-
-constexpr int NEAR_CH = NT;   // near evaluations per workgroup round (one per thread)
 
 // Evaluate the near pair of ball b and partner p with the reference arithmetic in the
 // reference orientation: A is the ball with the larger index in the caller's
@@ -509,124 +497,6 @@ __device__ __forceinline__ void eval_entry(const State& c, const int* perm, cons
     r.is_a = is_a;
 }
 
-// The near pairs of every tile whose masks are not all zero (one workgroup per tile;
-// the others exit at once).  Lane l of wavefront 0 owns the tile's i ball 64 I + l and
-// j ball 64 J + l; its entries are the partners of its i ball (in block J; on the
-// diagonal, all of its partners), then those of its j ball (in block I): every near
-// pair is evaluated by the lanes of both of its balls, identically.  The entries of
-// all lanes are numbered in lane order and evaluated NEAR_CH at a time, one per
-// thread of the workgroup; each lane of wavefront 0 then adds its own entries' results
-// in entry order and writes the near sums of its balls to their slots.
-template <bool WRITE>
-__global__ __launch_bounds__(NT) void k_near(TileArgs a)
-{
-    __shared__ NearRes res[NEAR_CH];
-    __shared__ int ent[NEAR_CH];
-    __shared__ int total_s;
-    const int tid = threadIdx.x, l = tid % TB;
-    const int t = blockIdx.x;
-    const int4 tt = a.tiles[t];
-    const int I = tt.x, J = tt.y, sys = tt.z, blk0 = tt.w;
-    const bool diag = I == J;
-    const size_t np = (size_t)a.n_pad;
-    const size_t si = (size_t)(J - blk0), sj_ = (size_t)(I - blk0);
-    const int i = I * TB + l, j0 = J * TB + l;
-    unsigned long long ma = 0, mb = 0;
-    int ca = 0, cnt = 0, base = 0;
-    if (tid < TB) {
-        ma = a.nbm[si * np + i];
-        mb = diag ? 0ull : a.nbm[sj_ * np + j0];
-        ca = __popcll(ma);
-        cnt = ca + __popcll(mb);
-        int incl = cnt;
-        for (int m = 1; m < TB; m <<= 1) {
-            const int y = __shfl_up(incl, m, TB);
-            if (l >= m) incl += y;
-        }
-        base = incl - cnt;
-        const int tot = __shfl(incl, TB - 1, TB);
-        if (tid == 0) total_s = tot;
-    }
-    __syncthreads();
-    const int total = total_s;
-    if (total == 0) {
-        if (WRITE && tid == 0) a.tile_nc[t] = 0;
-        return;   // uniform over the workgroup
-    }
-    const Phys& P = a.sysP[sys];
-    NearAcc ni = {{0, 0, 0}, {0, 0, 0}, 0, 0, 0}, nj = ni;
-    const State& c = a.cur;
-    const int Kp = diag ? I : J;
-    for (int c0 = 0; c0 < total; c0 += NEAR_CH) {
-        const int c1 = c0 + NEAR_CH;
-        if (tid < TB && cnt && base < c1 && base + cnt > c0) {
-            int k = base;
-            for (int side = 0; side < 2 && k < c1; side++) {
-                unsigned long long m = side ? mb : ma;
-                while (m && k < c1) {
-                    const int bit = __ffsll((long long)m) - 1;
-                    m &= m - 1;
-                    if (k >= c0) ent[k - c0] = l | (bit << 6) | (side << 12);
-                    k++;
-                }
-            }
-        }
-        __syncthreads();
-        if (tid < total - c0) {
-            const int e = ent[tid];
-            const int el = e & (TB - 1), bit = (e >> 6) & (TB - 1), side = e >> 12;
-            const int b = side ? J * TB + el : I * TB + el;
-            const int p = side ? I * TB + bit : Kp * TB + bit;
-            NearRes r;
-            eval_entry<WRITE>(c, a.perm, P, b, p, r);
-            res[tid] = r;
-        }
-        __syncthreads();
-        if (tid < TB && cnt && base < c1 && base + cnt > c0) {
-            const int k0 = base > c0 ? base : c0, k1 = base + cnt < c1 ? base + cnt : c1;
-            for (int k = k0; k < k1; k++) {
-                const NearRes& r = res[k - c0];
-                NearAcc& acc = (k - base < ca) ? ni : nj;
-                for (int q = 0; q < 3; q++) {
-                    acc.F[q] = acc.F[q] + r.f[q];
-                    acc.T[q] = acc.T[q] + r.f[3 + q];
-                }
-                acc.pe = acc.pe + r.pe;
-                acc.contacts += r.contact;
-                acc.pairs += r.is_a;
-            }
-        }
-        __syncthreads();
-    }
-    if (tid >= TB) return;
-    if (ma) {
-        double* d = a.nd + si * 6 * np + i;
-        for (int k = 0; k < 3; k++) {
-            d[k * np] = ni.F[k];
-            d[(3 + k) * np] = ni.T[k];
-        }
-    }
-    if (mb) {
-        double* d = a.nd + sj_ * 6 * np + j0;
-        for (int k = 0; k < 3; k++) {
-            d[k * np] = nj.F[k];
-            d[(3 + k) * np] = nj.T[k];
-        }
-    }
-    const int nc = wave_isum(ni.contacts + nj.contacts), nn = wave_isum(ni.pairs + nj.pairs);
-    if (WRITE) {
-        const double near_pe = wave_sum(ni.pe + nj.pe);
-        if (l == 0) {
-            a.pe_tile[t] = a.pe_tile[t] + near_pe;
-            a.tile_nc[t] = nc;
-        }
-    }
-    if (l == 0) {
-        a.tile_ctr[2 * t] += (unsigned long long)nc;
-        a.tile_ctr[2 * t + 1] += (unsigned long long)nn;
-    }
-}
-
 struct BallArgs {
     State cur, nxt;
     const Phys* sysP;
@@ -637,10 +507,11 @@ struct BallArgs {
     int n_used, n_pad;
     const double* part;
     const unsigned long long* nbm;
-    const double* nd;
+    const int* perm;    // engine slot -> the caller's ball index (fixes near-pair orientation)
     int kick, drift_next;
     double4* bnd;       // per 64-ball block: bounding box of the drifted positions (lo, hi)
     double* e_blk;      // FAST_E_STRIDE doubles per 64-ball block (write steps)
+    unsigned long long* blk_ctr;   // per 64-ball block: running sums of touching and near pairs
     int* flags;         // [1]: a non-finite state appeared
 };
 
@@ -650,15 +521,26 @@ constexpr int BPB = TB;              // balls per k_ball block
 constexpr int SLOTS_IN_FLIGHT = 8;   // slots per lane loaded together
 
 // One 64-ball block per workgroup; lane l of every wavefront works for ball l.
-// Wavefront w sums the slots K = w, w + 8, ... in order: far-field partials (reads
-// of 64 consecutive doubles; Kahan-compensated in the mixed mode) and, where the
-// partner mask is nonzero, the tile's near-pair sums.  The eight wavefronts' sums
-// are combined in LDS in a fixed order and wavefront 0 finishes the step for the 64
-// balls (coalesced stores).
+//
+// 1. Wavefront w sums the far-field partials of the slots K = w, w + 8, ... in order
+//    (reads of 64 consecutive doubles; Kahan-compensated in the mixed mode) and counts
+//    the near-partner bits of those slots.
+// 2. The block's near entries are numbered ball by ball (then by wavefront, slot and
+//    bit), and all 512 threads evaluate them, KB_T at a time, with the reference
+//    arithmetic and orientation.  Lane l of wavefront 0 adds its ball's results in
+//    entry order.  Each near pair is so evaluated once from each of its balls,
+//    identically, and no sum needs an atomic.
+// 3. Wavefront 0 combines the eight far-field sums in a fixed order and finishes the
+//    step for the 64 balls (coalesced stores).
 template <bool COMP, bool WRITE>
 __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
 {
-    __shared__ double sf[KB_W][9][TB];       // far force, near force, near torque
+    __shared__ double sf[KB_W][3][TB];       // far-field force sums per wavefront
+    __shared__ int s_cnt[KB_W][TB];          // near entries per (wavefront, ball)
+    __shared__ int s_off[KB_W][TB];          // first entry of (wavefront, ball)
+    __shared__ int s_total;
+    __shared__ int ent[KB_T];                // entries of the current round
+    __shared__ NearRes res[KB_T];
     const int w = threadIdx.x / TB, l = threadIdx.x % TB;
     const int i = blockIdx.x * TB + l;
     const int sys = a.blk_sys[blockIdx.x];
@@ -683,7 +565,7 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
     }
 
     double fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
-    double n[6] = {0, 0, 0, 0, 0, 0};
+    int cn = 0;   // near entries of this (wavefront, ball)
     for (int K0 = w; K0 < nblk; K0 += SLOTS_IN_FLIGHT * KB_W) {
         double q[SLOTS_IN_FLIGHT][3];
         unsigned long long mk[SLOTS_IN_FLIGHT];
@@ -697,11 +579,10 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
             q[u][2] = pp[2 * np];
             mk[u] = K < nblk ? a.nbm[(size_t)Kc * np + ic] : 0ull;
         }
-        unsigned nz = 0;
 #pragma unroll
         for (int u = 0; u < SLOTS_IN_FLIGHT; u++) {
             if (K0 + u * KB_W >= nblk) break;
-            if (mk[u]) nz |= 1u << u;
+            cn += __popcll(mk[u]);
             if (COMP) {
                 double y, t;
                 y = q[u][0] - cx; t = fx + y; cx = (t - fx) - y; fx = t;
@@ -713,19 +594,77 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
                 fz = fz + q[u][2];
             }
         }
-        // near sums of the slots whose tile found near pairs for this ball
-        while (nz) {
-            const int u = __ffs(nz) - 1;
-            nz &= nz - 1;
-            const double* d = a.nd + (size_t)(K0 + u * KB_W) * 6 * np + ic;
-            for (int k = 0; k < 6; k++) n[k] = n[k] + d[k * np];
-        }
     }
     sf[w][0][l] = fx - cx;
     sf[w][1][l] = fy - cy;
     sf[w][2][l] = fz - cz;
-    for (int k = 0; k < 6; k++) sf[w][3 + k][l] = n[k];
+    s_cnt[w][l] = cn;
     __syncthreads();
+
+    // number the block's near entries: ball-major, then wavefront, slot and bit
+    if (w == 0) {
+        int tot = 0;
+        for (int q = 0; q < KB_W; q++) tot += s_cnt[q][l];
+        int incl = tot;
+        for (int m = 1; m < TB; m <<= 1) {
+            const int y = __shfl_up(incl, m, TB);
+            if (l >= m) incl += y;
+        }
+        int o = incl - tot;
+        for (int q = 0; q < KB_W; q++) {
+            s_off[q][l] = o;
+            o += s_cnt[q][l];
+        }
+        if (l == TB - 1) s_total = incl;
+    }
+    __syncthreads();
+    const int total = s_total;
+
+    double n[6] = {0, 0, 0, 0, 0, 0};
+    double npe = 0;
+    int ncon = 0, npair = 0;
+    const int my0 = s_off[w][l];   // (wavefront, ball) entries are [my0, my0 + cn)
+    for (int r0 = 0; r0 < total; r0 += KB_T) {
+        const int r1 = r0 + KB_T;
+        // write this round's entries of (wavefront w, ball l)
+        if (cn && my0 < r1 && my0 + cn > r0) {
+            int k = my0;
+            for (int K = w; K < nblk && k < r1; K += KB_W) {
+                unsigned long long m = a.nbm[(size_t)K * np + ic];
+                while (m && k < r1) {
+                    const int bit = __ffsll((long long)m) - 1;
+                    m &= m - 1;
+                    if (k >= r0) ent[k - r0] = l | (bit << 6) | (K << 12);
+                    k++;
+                }
+            }
+        }
+        __syncthreads();
+        const int t = threadIdx.x;
+        if (r0 + t < total) {
+            const int e = ent[t];
+            const int el = e & (TB - 1), bit = (e >> 6) & (TB - 1), K = e >> 12;
+            NearRes rr;
+            eval_entry<WRITE>(c, a.perm, P, blockIdx.x * TB + el, off + K * TB + bit, rr);
+            res[t] = rr;
+        }
+        __syncthreads();
+        if (w == 0) {
+            // ball l's entries in this round, in order
+            int tot = 0;
+            for (int q = 0; q < KB_W; q++) tot += s_cnt[q][l];
+            const int b0 = s_off[0][l], b1 = b0 + tot;
+            const int k0 = b0 > r0 ? b0 : r0, k1 = b1 < r1 ? b1 : r1;
+            for (int k = k0; k < k1; k++) {
+                const NearRes& rr = res[k - r0];
+                for (int q = 0; q < 6; q++) n[q] = n[q] + rr.f[q];
+                npe = npe + rr.pe;
+                ncon += rr.contact;
+                npair += rr.is_a;
+            }
+        }
+        __syncthreads();
+    }
     if (w != 0) return;
 
     // wavefront 0: combine in a fixed order and finish the step of ball l
@@ -741,13 +680,8 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
                 s = s + sf[q][k][l];
             }
         }
-        double nsum = sf[0][3 + k][l], tsum = sf[0][6 + k][l];
-        for (int q = 1; q < KB_W; q++) {
-            nsum = nsum + sf[q][3 + k][l];
-            tsum = tsum + sf[q][6 + k][l];
-        }
-        F[k] = (s - cmp) + nsum;
-        T[k] = tsum;
+        F[k] = (s - cmp) + n[k];
+        T[k] = n[3 + k];
     }
 
     double e[FAST_E_STRIDE] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // PE KE p L contacts nonfinite
@@ -815,10 +749,21 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
     if (WRITE) {
         // a 64-ball block lies inside one system, so its sums are per system
         for (int k = 1; k < FAST_E_STRIDE; k++) {
+            if (k == 8) continue;
             const double s = wave_sum(e[k]);
             if (l == 0) a.e_blk[blockIdx.x * FAST_E_STRIDE + k] = s;
         }
-        if (l == 0) a.e_blk[blockIdx.x * FAST_E_STRIDE] = 0;
+    }
+    // the block's near-pair energy and pair counts (each pair counted by its ball A)
+    const double bpe = WRITE ? wave_sum(npe) : 0.0;
+    const int bcc = wave_isum(ncon), bcn = wave_isum(npair);
+    if (l == 0) {
+        a.blk_ctr[2 * blockIdx.x] += (unsigned long long)bcc;
+        a.blk_ctr[2 * blockIdx.x + 1] += (unsigned long long)bcn;
+        if (WRITE) {
+            a.e_blk[blockIdx.x * FAST_E_STRIDE] = bpe;
+            a.e_blk[blockIdx.x * FAST_E_STRIDE + 8] = (double)bcc;
+        }
     }
 }
 

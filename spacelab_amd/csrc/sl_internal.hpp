@@ -50,7 +50,7 @@ FastLayout fast_layout(int n_sys, const int* n_per_sys);
 
 // Optional hooks called around each kernel launch (profiling with HIP events).
 struct LaunchHook {
-    void (*before)(void* user, int kind) = nullptr;   // kind 0: pair/tile sweep, 1: per-ball, 2: near pairs
+    void (*before)(void* user, int kind) = nullptr;   // kind 0: pair/tile sweep, 1: per-ball kernel
     void (*after)(void* user, int kind) = nullptr;
     void* user = nullptr;
 };
@@ -86,13 +86,11 @@ struct FastBuffers {
     int n_tiles = 0, cap_tiles = 0;
     double* part = nullptr;       // far-field force partials [K][xyz][n_pad]
     unsigned long long* nbm = nullptr;   // near-partner masks [K][n_pad]
-    double* nd = nullptr;         // near-pair sums [K][F xyz, T xyz][n_pad] (where nbm != 0)
-    int part_slots = 0;           // K capacity of part, nbm and nd
+    int part_slots = 0;           // K capacity of part and nbm
     int* flags = nullptr;         // [1] non-finite state
-    int* tile_nc = nullptr;       // per tile: touching pairs (write steps)
-    int* h_tile_nc = nullptr;     // pinned host copy
-    unsigned long long* tile_ctr = nullptr;   // per tile: touching and near pairs (running sums)
-    double* e_blk = nullptr;      // per 64-ball block: (unused), KE, p, L, (unused), nonfinite
+    unsigned long long* blk_ctr = nullptr;    // per 64-ball block: touching and near pairs (running sums)
+    int cap_blocks = 0;
+    double* e_blk = nullptr;      // per 64-ball block: near-pair PE, KE, p, L, contacts, nonfinite
     double* pe_tile = nullptr;    // per tile: pair energy (write steps)
     double* h_e_blk = nullptr;    // pinned host copies
     double* h_pe_tile = nullptr;

@@ -7,10 +7,10 @@
 // are in Morton order of their positions at configuration; the caller's (canonical)
 // set is refreshed on demand.  Per step: k_tiles (the far-field sweep of every tile,
 // with the per-pair near test only where the blocks' bounding boxes come within h_min;
-// near pairs are recorded as partner masks), k_near (the near pairs of the tiles with
-// any, in the reference arithmetic), and k_ball (per-ball sums, kick, energies, the
-// next step's drift and block bounds).  Runs of steps between write steps are captured
-// once as HIP graphs and replayed.
+// near pairs are recorded as partner masks) and k_ball (per-ball sums of the far-field
+// partials and of the near pairs, evaluated there in the reference arithmetic, then
+// kick, energies, the next step's drift and block bounds).  Runs of steps between
+// write steps are captured once as HIP graphs and replayed.
 #include "spacelab/sl_api.h"
 #include "sl_fast_kernels.hpp"
 #include "sl_internal.hpp"
@@ -135,15 +135,6 @@ void launch_far_any(const TileArgs& a, int split, int precision, bool write, boo
         launch_far_split<1>(a, precision, write, grav, st);
 }
 
-void launch_near(const TileArgs& a, bool write, hipStream_t st)
-{
-    const int blocks = a.n_tiles;   // one workgroup per tile
-    if (write)
-        k_near<true><<<blocks, NT, 0, st>>>(a);
-    else
-        k_near<false><<<blocks, NT, 0, st>>>(a);
-}
-
 TileArgs tile_args(FastBuffers& fb)
 {
     const State& c = fb.eng[fb.cur];
@@ -163,12 +154,7 @@ TileArgs tile_args(FastBuffers& fb)
     a.Kf = fb.Kf;
     a.part = fb.part;
     a.nbm = fb.nbm;
-    a.nd = fb.nd;
     a.pe_tile = fb.pe_tile;
-    a.tile_nc = fb.tile_nc;
-    a.tile_ctr = fb.tile_ctr;
-    a.cur = c;
-    a.perm = fb.perm;
     return a;
 }
 
@@ -227,9 +213,6 @@ int launch_tiles_step(FastBuffers& fb, bool write, hipStream_t st, std::string* 
     if (fb.hook.before) fb.hook.before(fb.hook.user, 0);
     launch_far_any(a, fb.split, fb.precision, write, gravity_on(fb), st);
     if (fb.hook.after) fb.hook.after(fb.hook.user, 0);
-    if (fb.hook.before) fb.hook.before(fb.hook.user, 2);
-    launch_near(a, write, st);
-    if (fb.hook.after) fb.hook.after(fb.hook.user, 2);
     if (hipGetLastError() != hipSuccess) return err_hip(err, "tile kernel launch");
     return 0;
 }
@@ -248,11 +231,12 @@ int launch_ball_step(FastBuffers& fb, bool write, bool kick, bool drift_next, hi
     a.n_pad = fb.n_pad;
     a.part = fb.part;
     a.nbm = fb.nbm;
-    a.nd = fb.nd;
+    a.perm = fb.perm;
     a.kick = kick;
     a.drift_next = drift_next;
     a.bnd = fb.bnd;
     a.e_blk = fb.e_blk;
+    a.blk_ctr = fb.blk_ctr;
     a.flags = fb.flags;
     const int blocks = fb.lay.n_used / TB;
     if (fb.hook.before) fb.hook.before(fb.hook.user, 1);
@@ -283,12 +267,12 @@ int launch_drift_sys(FastBuffers& fb, hipStream_t st, std::string* err)
 }
 
 // Read back the partials of a write step and sum them per system, in a fixed order:
-// tiles in table order, then 64-ball blocks in ascending order.
+// the far-field energy of the tiles in table order, then the near-pair energy,
+// kinetic energy, momenta and contacts of the 64-ball blocks in ascending order.
 int read_energies(FastBuffers& fb, sl_energy* out, hipStream_t st, std::string* err)
 {
     const int nblk = fb.lay.n_used / BPB;
     if (hipMemcpyAsync(fb.h_pe_tile, fb.pe_tile, sizeof(double) * fb.n_tiles, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(fb.h_tile_nc, fb.tile_nc, sizeof(int) * fb.n_tiles, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipMemcpyAsync(fb.h_e_blk, fb.e_blk, sizeof(double) * FAST_E_STRIDE * nblk, hipMemcpyDeviceToHost, st) !=
             hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
@@ -296,14 +280,13 @@ int read_energies(FastBuffers& fb, sl_energy* out, hipStream_t st, std::string* 
     const int ns = fb.lay.n_sys;
     std::vector<sl_energy> e(ns);
     for (auto& x : e) x = sl_energy{};
-    for (int t = 0; t < fb.n_tiles; t++) {
-        e[fb.tile_sys_h[t]].PE += fb.h_pe_tile[t];
-        e[fb.tile_sys_h[t]].contacts += fb.h_tile_nc[t];
-    }
+    for (int t = 0; t < fb.n_tiles; t++) e[fb.tile_sys_h[t]].PE += fb.h_pe_tile[t];
     for (int b = 0; b < nblk; b++) {
         const int s = fb.blk_sys_h[(b * BPB) / TB];
         const double* v = fb.h_e_blk + (size_t)b * FAST_E_STRIDE;
         sl_energy& x = e[s];
+        x.PE += v[0];
+        x.contacts += (long long)v[8];
         x.KE += v[1];
         x.p[0] += v[2];
         x.p[1] += v[3];
@@ -483,6 +466,8 @@ int fast_alloc(FastBuffers& fb, int n_pad, int max_sys)
     if (hipMalloc(&fb.flags, sizeof(int) * 4) != hipSuccess) return -1;
     const int eblocks = (n_pad + BPB - 1) / BPB;
     if (hipMalloc(&fb.e_blk, sizeof(double) * FAST_E_STRIDE * eblocks) != hipSuccess) return -1;
+    if (hipMalloc(&fb.blk_ctr, sizeof(unsigned long long) * 2 * eblocks) != hipSuccess) return -1;
+    fb.cap_blocks = eblocks;
     if (hipHostMalloc(&fb.h_e_blk, sizeof(double) * FAST_E_STRIDE * eblocks, 0) != hipSuccess) return -1;
     if (hipMemset(fb.eng_base, 0, sizeof(double) * 2 * (size_t)NFIELD * n_pad) != hipSuccess) return -1;
     if (hipMemset(fb.flags, 0, sizeof(int) * 4) != hipSuccess) return -1;
@@ -503,13 +488,12 @@ void fast_free(FastBuffers& fb)
 {
     fast_clear_graphs(fb);
     void* ptrs[] = {fb.eng_base, fb.perm, fb.aux.q, fb.aux.g, fb.aux.q32, fb.aux.g32, fb.bnd, fb.blk_sys,
-                    fb.sys_off, fb.sys_n, fb.sys_nb, fb.sysP, fb.tiles, fb.part, fb.nbm, fb.nd, fb.flags,
-                    fb.tile_nc, fb.tile_ctr, fb.e_blk, fb.pe_tile};
+                    fb.sys_off, fb.sys_n, fb.sys_nb, fb.sysP, fb.tiles, fb.part, fb.nbm, fb.flags,
+                    fb.blk_ctr, fb.e_blk, fb.pe_tile};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (fb.h_e_blk) (void)hipHostFree(fb.h_e_blk);
     if (fb.h_pe_tile) (void)hipHostFree(fb.h_pe_tile);
-    if (fb.h_tile_nc) (void)hipHostFree(fb.h_tile_nc);
     fb = FastBuffers{};
 }
 
@@ -581,37 +565,25 @@ int fast_configure(FastBuffers& fb, State& canon, const FastLayout& lay, const P
         if (nt > fb.cap_tiles) {
             if (fb.tiles) (void)hipFree(fb.tiles);
             if (fb.pe_tile) (void)hipFree(fb.pe_tile);
-            if (fb.tile_nc) (void)hipFree(fb.tile_nc);
-            if (fb.tile_ctr) (void)hipFree(fb.tile_ctr);
             if (fb.h_pe_tile) (void)hipHostFree(fb.h_pe_tile);
-            if (fb.h_tile_nc) (void)hipHostFree(fb.h_tile_nc);
             fb.tiles = nullptr;
             fb.pe_tile = nullptr;
-            fb.tile_nc = nullptr;
-            fb.tile_ctr = nullptr;
             fb.h_pe_tile = nullptr;
-            fb.h_tile_nc = nullptr;
             fb.cap_tiles = 0;
             if (hipMalloc(&fb.tiles, sizeof(int4) * nt) != hipSuccess ||
                 hipMalloc(&fb.pe_tile, sizeof(double) * nt) != hipSuccess ||
-                hipMalloc(&fb.tile_nc, sizeof(int) * nt) != hipSuccess ||
-                hipMalloc(&fb.tile_ctr, sizeof(unsigned long long) * 2 * nt) != hipSuccess ||
-                hipHostMalloc(&fb.h_pe_tile, sizeof(double) * nt, 0) != hipSuccess ||
-                hipHostMalloc(&fb.h_tile_nc, sizeof(int) * nt, 0) != hipSuccess)
+                hipHostMalloc(&fb.h_pe_tile, sizeof(double) * nt, 0) != hipSuccess)
                 return err_hip(err, "tile table allocation");
             fb.cap_tiles = nt;
         }
         if (lay.nb_max > fb.part_slots) {
             if (fb.part) (void)hipFree(fb.part);
             if (fb.nbm) (void)hipFree(fb.nbm);
-            if (fb.nd) (void)hipFree(fb.nd);
             fb.part = nullptr;
             fb.nbm = nullptr;
-            fb.nd = nullptr;
             fb.part_slots = 0;
             if (hipMalloc(&fb.part, sizeof(double) * (size_t)lay.nb_max * 3 * fb.n_pad) != hipSuccess ||
-                hipMalloc(&fb.nbm, sizeof(unsigned long long) * (size_t)lay.nb_max * fb.n_pad) != hipSuccess ||
-                hipMalloc(&fb.nd, sizeof(double) * (size_t)lay.nb_max * 6 * fb.n_pad) != hipSuccess)
+                hipMalloc(&fb.nbm, sizeof(unsigned long long) * (size_t)lay.nb_max * fb.n_pad) != hipSuccess)
                 return err_hip(err, "partial buffer allocation");
             fb.part_slots = lay.nb_max;
         }
@@ -626,7 +598,7 @@ int fast_configure(FastBuffers& fb, State& canon, const FastLayout& lay, const P
         fb.n_tiles = nt;
         fb.split = 0;   // re-tune for the new sizes
     }
-    if (launch_zero_u64(fb.tile_ctr, 2 * fb.cap_tiles, st) != hipSuccess ||
+    if (launch_zero_u64(fb.blk_ctr, 2 * fb.cap_blocks, st) != hipSuccess ||
         launch_zero_i32(fb.flags, 4, st) != hipSuccess)
         return err_hip(err, "clear counters");
     // Morton order of the current positions, per system
@@ -729,19 +701,20 @@ int fast_steps(FastBuffers& fb, State& canon, int n_steps, long long* step_count
 
 hipError_t fast_reset_counters(FastBuffers& fb, hipStream_t st)
 {
-    if (!fb.tile_ctr) return hipSuccess;
-    return launch_zero_u64(fb.tile_ctr, 2 * fb.cap_tiles, st);
+    if (!fb.blk_ctr) return hipSuccess;
+    return launch_zero_u64(fb.blk_ctr, 2 * fb.cap_blocks, st);
 }
 
 int fast_read_counters(FastBuffers& fb, hipStream_t st, long long* contacts, long long* near_pairs)
 {
-    std::vector<unsigned long long> h(2 * (size_t)fb.n_tiles);
+    // running sums of the 64-ball blocks in use (each pair is counted by its ball A)
+    std::vector<unsigned long long> h(2 * (size_t)(fb.lay.n_used / BPB));
     if (h.empty()) {
         if (contacts) *contacts = 0;
         if (near_pairs) *near_pairs = 0;
         return 0;
     }
-    if (hipMemcpyAsync(h.data(), fb.tile_ctr, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, st) !=
+    if (hipMemcpyAsync(h.data(), fb.blk_ctr, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, st) !=
             hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         return SL_E_HIP;
