@@ -43,6 +43,9 @@ namespace fastk {
 
 constexpr int TB = FAST_TILE;
 constexpr int NW = 4;                 // wavefronts per block
+#ifndef SL_TILE_UNROLL
+#define SL_TILE_UNROLL 2              // pairs of the tile sweep unrolled per wavefront
+#endif
 constexpr int NT = TB * NW;
 
 // Lane l receives the value held by lane (l + 1) mod 64 (DPP wave_rol:1).
@@ -303,7 +306,7 @@ __global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
                     if constexpr (WRITE) {
                         for (int k = 0; k < NS; k++) pair(s0 + k, true, ntest);
                     } else {
-#pragma unroll 8
+#pragma unroll SL_TILE_UNROLL
                         for (int k = 0; k < NS; k++) pair(s0 + k, true, ntest);
                     }
                 } else {
@@ -365,7 +368,7 @@ __global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
                     if constexpr (WRITE) {
                         for (int k = 0; k < NS; k++) pair(s0 + k, true, ntest);
                     } else {
-#pragma unroll 8
+#pragma unroll SL_TILE_UNROLL
                         for (int k = 0; k < NS; k++) pair(s0 + k, true, ntest);
                     }
                 } else {
@@ -565,7 +568,9 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
     }
 
     double fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
-    int cn = 0;   // near entries of this (wavefront, ball)
+    int cn = 0;                    // near entries of this (wavefront, ball)
+    unsigned long long nzs = 0;    // bit o: slot w + 8 o has near partners
+    bool many = false;             // more than 64 slots per wavefront: rescan them all
     for (int K0 = w; K0 < nblk; K0 += SLOTS_IN_FLIGHT * KB_W) {
         double q[SLOTS_IN_FLIGHT][3];
         unsigned long long mk[SLOTS_IN_FLIGHT];
@@ -583,6 +588,13 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
         for (int u = 0; u < SLOTS_IN_FLIGHT; u++) {
             if (K0 + u * KB_W >= nblk) break;
             cn += __popcll(mk[u]);
+            if (mk[u]) {
+                const int o = (K0 - w) / KB_W + u;
+                if (o < 64)
+                    nzs |= 1ull << o;
+                else
+                    many = true;
+            }
             if (COMP) {
                 double y, t;
                 y = q[u][0] - cx; t = fx + y; cx = (t - fx) - y; fx = t;
@@ -629,7 +641,7 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
         // write this round's entries of (wavefront w, ball l)
         if (cn && my0 < r1 && my0 + cn > r0) {
             int k = my0;
-            for (int K = w; K < nblk && k < r1; K += KB_W) {
+            auto put = [&](int K) {
                 unsigned long long m = a.nbm[(size_t)K * np + ic];
                 while (m && k < r1) {
                     const int bit = __ffsll((long long)m) - 1;
@@ -637,6 +649,12 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
                     if (k >= r0) ent[k - r0] = l | (bit << 6) | (K << 12);
                     k++;
                 }
+            };
+            if (!many) {
+                // only the slots that had partners (their masks are still in cache)
+                for (unsigned long long z = nzs; z && k < r1; z &= z - 1) put(w + (__ffsll((long long)z) - 1) * KB_W);
+            } else {
+                for (int K = w; K < nblk && k < r1; K += KB_W) put(K);
             }
         }
         __syncthreads();

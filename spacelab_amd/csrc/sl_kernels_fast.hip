@@ -167,9 +167,10 @@ bool split_ready(const FastBuffers& fb) { return fb.split != 0 && fb.split_key =
 
 // Autotuner: the number of wavefronts sharing one far tile trades per-wave set-up
 // against occupancy and load balance, which depend on the systems' sizes and the
-// machine.  Each choice is warmed up once and then timed over five launches on the
-// live state (HIP events on the engine's stream); the fastest is kept.  The trial
-// launches only write partial sums that the real launch overwrites.
+// machine.  Each choice is warmed up once and then timed over three rounds of five
+// launches on the live state (HIP events on the engine's stream); the choice with the
+// fastest round is kept.  The trial launches only write partial sums that the real
+// launch overwrites.
 int tune_split(FastBuffers& fb, const TileArgs& a, hipStream_t st)
 {
     if (fb.forced_split == 1 || fb.forced_split == 2 || fb.forced_split == 4) return fb.forced_split;
@@ -183,19 +184,20 @@ int tune_split(FastBuffers& fb, const TileArgs& a, hipStream_t st)
     int best = 1;
     float best_ms = 1e30f;
     const int choices[3] = {1, 2, 4};
-    for (int c : choices) {
-        launch_far_any(a, c, fb.precision, false, grav, st);
-        (void)hipEventRecord(e0, st);
-        for (int rep = 0; rep < 5; rep++) launch_far_any(a, c, fb.precision, false, grav, st);
-        (void)hipEventRecord(e1, st);
-        if (hipEventSynchronize(e1) != hipSuccess) break;
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = c;
+    for (int round = 0; round < 3; round++)
+        for (int c : choices) {
+            launch_far_any(a, c, fb.precision, false, grav, st);
+            (void)hipEventRecord(e0, st);
+            for (int rep = 0; rep < 5; rep++) launch_far_any(a, c, fb.precision, false, grav, st);
+            (void)hipEventRecord(e1, st);
+            if (hipEventSynchronize(e1) != hipSuccess) break;
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
         }
-    }
     (void)hipStreamSynchronize(st);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
