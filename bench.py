@@ -202,6 +202,34 @@ def ensemble(n_coll=512, steps=200, warmup=20, horizon=50000, seed=3):
                     "are a bounded sample of the 50k-step run, inputs resident in HBM"}
 
 
+def mlp_only(epochs=10):
+    """The surrogate network alone: training on the stored collisions by the library's
+    kernels and inference on the held-out descriptors (no DEM)."""
+    from spacelab_amd import surrogate as S
+    data = os.path.join(ROOT, "spacelab_amd", "data")
+    train = S.load_dataset(os.path.join(data, "collisions_train.npz"))
+    held = S.load_dataset(os.path.join(data, "collisions_heldout.npz"))
+    sur = S.Surrogate(n_models=4)
+    try:
+        t0 = time.perf_counter()
+        sur.fit(train["X"], train["cls"], train["reg"], epochs=epochs)
+        t_train = time.perf_counter() - t0
+        sur.predict(held["X"])
+        t0 = time.perf_counter()
+        reps = 20
+        for _ in range(reps):
+            cls, _, _, _ = sur.predict(held["X"])
+        t_inf = (time.perf_counter() - t0) / reps
+        acc = float((cls == held["cls"]).mean())
+    finally:
+        sur.close()
+    n = len(train["cls"])
+    steps = epochs * (n // 256) * 4
+    return {"train_s": t_train, "train_steps": steps, "ms_per_train_step": 1e3 * t_train / steps,
+            "inference_s_per_2000": t_inf, "collisions_per_s_inference": len(held["cls"]) / t_inf,
+            "raw_accuracy": acc, "models": 4, "epochs": epochs}
+
+
 def surrogate_eval():
     """Surrogate plus DEM fallback against DEM alone on the 2000 held-out collisions
     (reduced scale: targets 32-512 balls, projectiles 1-64, 40000 steps at 100-fold
@@ -231,7 +259,7 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=500)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["ensemble", "surrogate"], default=None,
+    ap.add_argument("--only", choices=["ensemble", "surrogate", "mlp"], default=None,
                     help="run only this part and print its object (for profiling)")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
@@ -246,7 +274,8 @@ def main():
 
     import spacelab_amd as sl
     if a.only:
-        out = ensemble(steps=a.ensemble_steps) if a.only == "ensemble" else surrogate_eval()
+        out = {"ensemble": lambda: ensemble(steps=a.ensemble_steps), "surrogate": surrogate_eval,
+               "mlp": mlp_only}[a.only]()
         print(json.dumps({a.only: out}), flush=True)
         return
     if a.copy_probe_mb > 0:
