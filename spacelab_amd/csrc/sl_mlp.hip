@@ -26,8 +26,9 @@ typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 constexpr int NL = 4;          // layers
-constexpr int TILE = 64;       // GEMM block tile (M and N)
-constexpr int KT = 16;         // GEMM K step (one MFMA)
+constexpr int GM = 32;         // GEMM block tile rows
+constexpr int GN = 32;         // GEMM block tile columns
+constexpr int GK = 64;         // GEMM K step staged in LDS
 constexpr int NCLS = 3;
 
 __host__ __device__ inline bf16 to_bf16(float f)
@@ -50,63 +51,72 @@ __host__ __device__ inline float from_bf16(bf16 h)
 // C[M][N] = op(A)[M][K] * op(B)[K][N] (+ bias[N]) with bfloat16 operands and binary32
 // accumulation.  op(A)(m, k) = TA ? A[k * lda + m] : A[m * lda + k]; op(B)(k, n) =
 // TB ? B[n * ldb + k] : B[k * ldb + n].  Optional ReLU; the result is stored in binary32
-// (C) and/or bfloat16 (Cb).  A 256-thread block computes a 64 x 64 tile: wavefront w
-// owns the 32 x 32 quarter (w / 2, w % 2) as 2 x 2 MFMA 16 x 16 tiles.  Operand tiles
-// are staged through LDS with zero fill outside the matrices.
+// (C) and/or bfloat16 (Cb).  A 256-thread block computes a 32 x 32 tile: wavefront w
+// owns the 16 x 16 quarter (w / 2, w % 2), one v_mfma_f32_16x16x16bf16_1k per 16 of K.
+// Operand tiles of GK = 64 along K are staged through LDS (zero fill outside the
+// matrices); global reads run along each matrix's contiguous index, and both tiles are
+// stored K-contiguous so each lane reads its four K values as one 8-byte word.  The
+// small output tile keeps many workgroups in flight for the 256-wide layers.
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16* A, int lda, const bf16* B, int ldb,
                                               float* C, int ldc, bf16* Cb, int ldcb, const float* bias, int relu)
 {
-    __shared__ bf16 As[TILE][KT + 4];
-    __shared__ bf16 Bs[KT][TILE + 4];
+    __shared__ __attribute__((aligned(16))) bf16 As[GM][GK + 8];   // [m][k]
+    __shared__ __attribute__((aligned(16))) bf16 Bt[GN][GK + 8];   // [n][k]
     const int tid = threadIdx.x, w = tid / 64, l = tid % 64;
-    const int m0 = blockIdx.y * TILE, n0 = blockIdx.x * TILE;
-    const int wm = (w / 2) * 32, wn = (w % 2) * 32;
-    float4_t acc[2][2];
-    for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += KT) {
-        for (int e = tid; e < TILE * KT; e += 256) {
-            const int r = e / KT, c = e % KT;
+    const int m0 = blockIdx.y * GM, n0 = blockIdx.x * GN;
+    const int wm = (w / 2) * 16, wn = (w % 2) * 16;
+    float4_t acc = float4_t{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += GK) {
+        for (int e = tid; e < GM * GK; e += 256) {
+            int r, c;   // r: m within the tile, c: k within the step
+            if (TA) {
+                c = e / GM;
+                r = e % GM;
+            } else {
+                r = e / GK;
+                c = e % GK;
+            }
             const int gm = m0 + r, gk = k0 + c;
             bf16 v = 0;
             if (gm < M && gk < K) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
             As[r][c] = v;
         }
-        for (int e = tid; e < KT * TILE; e += 256) {
-            const int r = e / TILE, c = e % TILE;
+        for (int e = tid; e < GK * GN; e += 256) {
+            int r, c;   // r: k within the step, c: n within the tile
+            if (TB) {
+                c = e / GK;
+                r = e % GK;
+            } else {
+                r = e / GN;
+                c = e % GN;
+            }
             const int gk = k0 + r, gn = n0 + c;
             bf16 v = 0;
             if (gk < K && gn < N) v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
-            Bs[r][c] = v;
+            Bt[c][r] = v;
         }
         __syncthreads();
-        const int kq = 4 * (l / 16);
-        for (int i = 0; i < 2; i++) {
-            const int row = wm + i * 16 + (l % 16);
-            short4_t a;
-            for (int q = 0; q < 4; q++) a[q] = (short)As[row][kq + q];
-            for (int j = 0; j < 2; j++) {
-                const int col = wn + j * 16 + (l % 16);
-                short4_t b;
-                for (int q = 0; q < 4; q++) b[q] = (short)Bs[kq + q][col];
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i][j], 0, 0, 0);
-            }
+        const int row = wm + (l % 16), col = wn + (l % 16);
+#pragma unroll
+        for (int kk = 0; kk < GK; kk += 16) {
+            const int kq = kk + 4 * (l / 16);
+            const short4_t a = *reinterpret_cast<const short4_t*>(&As[row][kq]);
+            const short4_t b = *reinterpret_cast<const short4_t*>(&Bt[col][kq]);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
         }
         __syncthreads();
     }
-    for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++)
-            for (int r = 0; r < 4; r++) {
-                const int row = m0 + wm + i * 16 + 4 * (l / 16) + r;
-                const int col = n0 + wn + j * 16 + (l % 16);
-                if (row >= M || col >= N) continue;
-                float v = acc[i][j][r];
-                if (bias) v += bias[col];
-                if (relu && v < 0.f) v = 0.f;
-                if (C) C[(size_t)row * ldc + col] = v;
-                if (Cb) Cb[(size_t)row * ldcb + col] = to_bf16(v);
-            }
+    for (int r = 0; r < 4; r++) {
+        const int row = m0 + wm + 4 * (l / 16) + r;
+        const int col = n0 + wn + (l % 16);
+        if (row >= M || col >= N) continue;
+        float v = acc[r];
+        if (bias) v += bias[col];
+        if (relu && v < 0.f) v = 0.f;
+        if (C) C[(size_t)row * ldc + col] = v;
+        if (Cb) Cb[(size_t)row * ldcb + col] = to_bf16(v);
+    }
 }
 
 __global__ void k_f32_to_bf16(int n, const float* x, bf16* y)
@@ -165,14 +175,20 @@ __global__ void k_relu_back(int n, const float* z, float* dz, bf16* dzb)
     dzb[i] = to_bf16(g);
 }
 
-// Column sums of dz[B][ld] for the first n columns (bias gradient), fixed order.
-__global__ void k_colsum(int B, int n, int ld, const float* dz, float* db)
+// Column sums of dz[B][ld] for the first n columns (bias gradient) in a fixed order:
+// 64 columns per 256-thread block, row phase p = 0..3 of each column summing rows
+// p, p + 4, ... in order, then the four phases added in order.
+__global__ __launch_bounds__(256) void k_colsum(int B, int n, int ld, const float* dz, float* db)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
+    __shared__ float part[4][64];
+    const int cl = threadIdx.x % 64, ph = threadIdx.x / 64;
+    const int c = blockIdx.x * 64 + cl;
     float s = 0.f;
-    for (int r = 0; r < B; r++) s += dz[(size_t)r * ld + c];
-    db[c] = s;
+    if (c < n)
+        for (int r = ph; r < B; r += 4) s += dz[(size_t)r * ld + c];
+    part[ph][cl] = s;
+    __syncthreads();
+    if (ph == 0 && c < n) db[c] = ((part[0][cl] + part[1][cl]) + part[2][cl]) + part[3][cl];
 }
 
 // Adam (Kingma & Ba 2015) on binary32 master weights, refreshing the bfloat16 copy.
@@ -255,7 +271,7 @@ template <bool TA, bool TB>
 void gemm(hipStream_t st, int M, int N, int K, const bf16* A, int lda, const bf16* B, int ldb, float* C, int ldc,
           bf16* Cb, int ldcb, const float* bias, bool relu)
 {
-    const dim3 grid((N + TILE - 1) / TILE, (M + TILE - 1) / TILE);
+    const dim3 grid((N + GN - 1) / GN, (M + GM - 1) / GM);
     k_gemm<TA, TB><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, Cb, ldcb, bias, relu ? 1 : 0);
 }
 
@@ -301,7 +317,7 @@ void backward(sl_mlp* m, int B)
         const bf16* Hprev = k ? m->Hb[k - 1] : m->xb;
         gemm<true, false>(m->stream, L.ldin, L.ldout, B, Hprev, L.ldin, m->dZb[k], L.ldout, L.gW, L.ldout, nullptr, 0,
                           nullptr, false);
-        k_colsum<<<(L.ldout + 255) / 256, 256, 0, m->stream>>>(B, L.ldout, L.ldout, m->dZ[k], L.gb);
+        k_colsum<<<(L.ldout + 63) / 64, 256, 0, m->stream>>>(B, L.ldout, L.ldout, m->dZ[k], L.gb);
         if (k > 0) {
             gemm<false, true>(m->stream, B, L.ldin, L.ldout, m->dZb[k], L.ldout, L.Wb, L.ldout, m->dZ[k - 1], L.ldin,
                               nullptr, 0, nullptr, false);
