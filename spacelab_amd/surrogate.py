@@ -64,6 +64,7 @@ class DemSettings:
     precision: int = PREC_FP64
     max_systems: int = 512      # collisions per batched launch
     preset: str = "config"
+    tile_split: int = 2         # fixed, so a collision's result does not depend on its batch
 
 
 def _log_uniform_int(rng, lo, hi, n):
@@ -119,29 +120,43 @@ class DemRunner:
         self.s = s
         p = preset(s.preset).replace(order=ORDER_FAST, precision=s.precision)
         self.batch = Batch(s.max_systems, max_balls, p)
+        self.batch.set_option("tile_split", s.tile_split)
 
     def close(self):
         self.batch.close()
 
-    def run(self, preps: list[Prepared], steps: int | None = None) -> tuple[np.ndarray, np.ndarray, list]:
+    def run(self, preps: list[Prepared], steps: int | None = None, looks: tuple = ()):
         """(class labels, regression targets, outcomes) of the collisions after
-        ``steps`` (default: the settings') DEM steps."""
+        ``steps`` (default: the settings') DEM steps.  With ``looks`` (step counts below
+        ``steps``) it returns a fourth item: {step: look-ahead features} of the same
+        trajectories at those steps."""
         s = self.s
         steps = s.steps if steps is None else steps
+        looks = sorted(h for h in looks if 0 < h < steps)
         labels, regs, outs = [], [], []
+        feats = {h: [] for h in looks}
         for c0 in range(0, len(preps), s.max_systems):
             chunk = preps[c0:c0 + s.max_systems]
             b = self.batch
             b.upload([q.setup.group for q in chunk], kin=[q.kin for q in chunk], kout=[q.kout for q in chunk],
                      dt=[q.dt for q in chunk])
             b.init_conditions()
-            b.step(steps)
+            done = 0
+            for h in looks:
+                b.step(h - done)
+                done = h
+                for q, g in zip(chunk, b.download()):
+                    feats[h].append(look_row(collision_outcome(q.setup.n_t, g, s.link * s.r_med), q.setup.speed))
+            b.step(steps - done)
             for q, g in zip(chunk, b.download()):
                 o = collision_outcome(q.setup.n_t, g, s.link * s.r_med)
                 labels.append(o.label)
                 regs.append(targets(o, q.setup.speed))
                 outs.append(o)
-        return np.array(labels, dtype=np.int32), np.array(regs, dtype=np.float32).reshape(-1, N_REG), outs
+        res = (np.array(labels, dtype=np.int32), np.array(regs, dtype=np.float32).reshape(-1, N_REG), outs)
+        if looks:
+            return res + ({h: np.array(v, dtype=np.float32).reshape(-1, N_LOOK) for h, v in feats.items()},)
+        return res
 
 
 @dataclass
@@ -169,13 +184,14 @@ class Surrogate:
     seed: int = 1
     n_models: int = 4
     max_batch: int = 4096
+    n_in: int = N_DESCRIPTORS
     threshold: float = float("inf")
     norm: Normaliser | None = None
     mlps: list = field(default_factory=list, repr=False)
 
     def __post_init__(self):
         if not self.mlps:
-            self.mlps = [MLP(N_DESCRIPTORS, 3 + N_REG, self.width, self.max_batch, self.seed * 1000 + k)
+            self.mlps = [MLP(self.n_in, 3 + N_REG, self.width, self.max_batch, self.seed * 1000 + k)
                          for k in range(self.n_models)]
 
     def close(self):
@@ -330,25 +346,33 @@ def evaluate(train: dict, heldout: dict, target: float = 0.95, z: float = 2.0, v
 N_T_RANGE, N_P_RANGE, SPEED_RANGE = (32, 512), (1, 64), (10.0, 1000.0)
 
 
-def make_dataset(count: int, seed: int, s: DemSettings, log=None) -> dict:
-    """DEM-labelled collisions: descriptors, classes, regression targets and the specs
-    that regenerate each set-up."""
+LOOK_STEPS = (4000, 8000)
+
+
+def make_dataset(count: int, seed: int, s: DemSettings, log=None, looks=LOOK_STEPS) -> dict:
+    """DEM-labelled collisions: descriptors, classes, regression targets, look-ahead
+    features of the same trajectories ("L<steps>") and the specs that regenerate each
+    set-up."""
     specs = sample_specs(count, seed, N_T_RANGE, N_P_RANGE, SPEED_RANGE)
     dem = DemRunner(s, s.max_systems * (N_T_RANGE[1] + N_P_RANGE[1]))
     X, C, Rg, met = [], [], [], []
+    Ls = {h: [] for h in looks}
     t0 = time.perf_counter()
     try:
         for c0 in range(0, count, s.max_systems):
             preps = [prepare(q, s) for q in specs[c0:c0 + s.max_systems]]
             X.append(descriptors(preps, s))
-            c, r, _ = dem.run(preps)
+            c, r, _, lf = dem.run(preps, looks=looks)
+            for h in looks:
+                Ls[h].append(lf[h])
             C.append(c); Rg.append(r); met += [q.met for q in preps]
             if log:
                 log(f"{c0 + len(preps)}/{count} collisions, {time.perf_counter() - t0:.1f} s, "
                     f"classes so far {np.bincount(np.concatenate(C), minlength=3).tolist()}")
     finally:
         dem.close()
-    return {"X": np.concatenate(X), "cls": np.concatenate(C), "reg": np.concatenate(Rg),
+    out = {f"L{h}": np.concatenate(v) for h, v in Ls.items()}
+    return out | {"X": np.concatenate(X), "cls": np.concatenate(C), "reg": np.concatenate(Rg),
             "met": np.array(met), "n_t": np.array([q.n_t for q in specs]),
             "n_p": np.array([q.n_p for q in specs]), "seed": np.array([q.seed for q in specs], dtype=np.uint64),
             "speed": np.array([q.speed for q in specs]), "b_frac": np.array([q.b_frac for q in specs]),
@@ -374,6 +398,24 @@ def settings_of(ds: dict) -> DemSettings:
     return DemSettings(**json.loads(str(ds["settings"])))
 
 
+# ---- look-ahead stage: the provisional outcome after a short DEM run -------------
+# 12 features of the outcome at the look-ahead step: the provisional class (one-hot),
+# ln(components), the two largest bodies' mass fractions, the mass transfer, the x and y
+# centre-of-mass velocities of the two largest bodies over the impact speed, and their
+# relative speed over the impact speed.
+N_LOOK = 12
+
+
+def look_row(o, speed: float) -> np.ndarray:
+    v1 = np.array(o.v_largest[:]) / speed
+    v2 = np.array(o.v_second[:]) / speed
+    onehot = [1.0 if o.label == k else 0.0 for k in range(3)]
+    return np.array(onehot + [math.log(max(o.n_components, 1)), o.largest_mass_frac, o.second_mass_frac,
+                              o.mass_transfer, v1[0], v1[1], v2[0], v2[1],
+                              float(np.linalg.norm(v1 - v2)) if o.n_components > 1 else 0.0])
+
+
+
 def load_dataset(path: str) -> dict:
     with np.load(path, allow_pickle=False) as f:
         return {k: f[k] for k in f.files}
@@ -385,10 +427,12 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--steps", type=int, default=DemSettings.steps)
     ap.add_argument("--softening", type=float, default=DemSettings.softening)
+    ap.add_argument("--look-steps", default=",".join(str(h) for h in LOOK_STEPS))
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)
     s = DemSettings(steps=a.steps, softening=a.softening)
-    ds = make_dataset(a.count, a.seed, s, log=lambda m: print(m, flush=True))
+    ds = make_dataset(a.count, a.seed, s, log=log, looks=tuple(int(x) for x in a.look_steps.split(",")))
     np.savez_compressed(a.out, **ds)
     print(f"wrote {a.out}: classes {np.bincount(ds['cls'], minlength=3).tolist()}", flush=True)
 
