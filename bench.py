@@ -211,6 +211,41 @@ def ensemble(n_coll=512, steps=200, warmup=20, horizon=50000, seed=3):
                     "are a bounded sample of the 50k-step run, inputs resident in HBM"}
 
 
+def collision(steps=50000, seed=7, n_t=2000, n_p=200):
+    """One aggregate collision run in full: seeded BPCA target (2000 spheres) and projectile
+    (200, independent seed), random orientations, impact speed uniform in 10-1000 cm/s and
+    impact parameter uniform in [0, 1] (R_t + R_p) (drawn from ``seed``), the projectile
+    moved to first contact minus 0.05 median radii, springs and dt calibrated at the impact
+    speed (updateDTK's kin, t_c/50).  Plain FP64 and mixed precision from the same state."""
+    import spacelab_amd as sl
+    rng = np.random.default_rng(seed)
+    speed, b = float(rng.uniform(10.0, 1000.0)), float(rng.uniform(0.0, 1.0))
+    c = sl.generate_collision(n_t, n_p, seed, speed, b)
+    sl.collision_approach(c, 0.05 * 1e-5)
+    g = c.group
+    p = sl.calibrate(sl.preset("config"), g.R, g.m, velocity=speed, cor=0.64, dt_rule="tc50")
+    out = {"n_t": n_t, "n_p": n_p, "speed": speed, "b_frac": b, "steps": steps, "dt": p.dt}
+    ends = {}
+    for name, prec in (("fp64", sl.PREC_FP64), ("mixed", sl.PREC_MIXED)):
+        with sl.Collider(n_t + n_p, p.replace(order=sl.ORDER_FAST, precision=prec)) as col:
+            col.upload(g)
+            e0 = col.init_conditions()
+            col.synchronize()
+            t0 = time.perf_counter()
+            ew = col.step(steps, steps)
+            col.synchronize()
+            sec = time.perf_counter() - t0
+            fin = col.download()
+        o = sl.collision_outcome(n_t, fin, 0.1 * 1e-5)
+        ends[name] = ew[-1]
+        out[name] = {"seconds": sec, "us_per_step": 1e6 * sec / steps, "collisions_per_s": 1.0 / sec,
+                     "outcome": ["stick", "bounce", "fragment"][o.label], "n_components": o.n_components,
+                     "largest_mass_frac": o.largest_mass_frac,
+                     "energy_change_rel": abs(ew[-1].E - e0.E) / energy_scale(e0)}
+    out["mixed_vs_fp64_energy_rel"] = abs(ends["mixed"].E - ends["fp64"].E) / energy_scale(ends["fp64"])
+    return out
+
+
 def mlp_only(epochs=10):
     """The surrogate network alone: training on the stored collisions by the library's
     kernels and inference on the held-out descriptors (no DEM)."""
@@ -268,10 +303,11 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=500)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["ensemble", "surrogate", "mlp"], default=None,
+    ap.add_argument("--only", choices=["ensemble", "surrogate", "mlp", "collision"], default=None,
                     help="run only this part and print its object (for profiling)")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-collision", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run the plain copy kernel on this many MiB first (counter calibration)")
@@ -284,7 +320,7 @@ def main():
     import spacelab_amd as sl
     if a.only:
         out = {"ensemble": lambda: ensemble(steps=a.ensemble_steps), "surrogate": surrogate_eval,
-               "mlp": mlp_only}[a.only]()
+               "mlp": mlp_only, "collision": collision}[a.only]()
         print(json.dumps({a.only: out}), flush=True)
         return
     if a.copy_probe_mb > 0:
@@ -387,6 +423,11 @@ def main():
         log(f"cpu baseline: oracle ordered gather on {threads} threads")
         extra["cpu_omp"] = cpu_omp(g, p, threads)
 
+    if not a.no_collision:
+        log("collision: 2000 + 200 spheres, 50000 steps, fp64 and mixed")
+        extra["collision"] = collision()
+        log(f"collision: fp64 {extra['collision']['fp64']['seconds']:.2f} s, "
+            f"mixed {extra['collision']['mixed']['seconds']:.2f} s, outcome {extra['collision']['fp64']['outcome']}")
     if not a.no_ensemble:
         log(f"ensemble: 512 collisions at full size, {a.ensemble_steps} timed steps")
         extra["ensemble"] = ensemble(steps=a.ensemble_steps)
