@@ -555,7 +555,13 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
     const State& c = a.cur;
     const size_t np = (size_t)a.n_pad;
 
-    // wavefront 0 finishes the step: its own-state loads go out first
+    // wavefront 0 finishes the step: its own-state loads (and the block's running pair
+    // counters) go out first
+    unsigned long long ctr0 = 0, ctr1 = 0;
+    if (w == 0 && l == 0) {
+        ctr0 = a.blk_ctr[2 * blockIdx.x];
+        ctr1 = a.blk_ctr[2 * blockIdx.x + 1];
+    }
     D3 p0 = d3(0, 0, 0), h0 = p0, g0 = p0;
     double mi = 1, Ii = 1, Rb = 0;
     if (w == 0 && valid) {
@@ -776,8 +782,8 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
     const double bpe = WRITE ? wave_sum(npe) : 0.0;
     const int bcc = wave_isum(ncon), bcn = wave_isum(npair);
     if (l == 0) {
-        a.blk_ctr[2 * blockIdx.x] += (unsigned long long)bcc;
-        a.blk_ctr[2 * blockIdx.x + 1] += (unsigned long long)bcn;
+        a.blk_ctr[2 * blockIdx.x] = ctr0 + (unsigned long long)bcc;
+        a.blk_ctr[2 * blockIdx.x + 1] = ctr1 + (unsigned long long)bcn;
         if (WRITE) {
             a.e_blk[blockIdx.x * FAST_E_STRIDE] = bpe;
             a.e_blk[blockIdx.x * FAST_E_STRIDE + 8] = (double)bcc;

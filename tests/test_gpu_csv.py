@@ -88,3 +88,25 @@ def test_driver_rejects_bad_input(tmp_path):
     r = subprocess.run([sl.collider_program(), "--continue", str(tmp_path) + "/", "nothing_"],
                        capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "cannot" in r.stderr
+
+
+def test_driver_generate_mode(tmp_path):
+    """--generate builds a seeded aggregate and runs it; the files have the reference's
+    layout: a header and one row per write step plus the initial row."""
+    r = subprocess.run([sl.collider_program(), "--generate", "100", "0x5EED", "--out", str(tmp_path) + "/gen_",
+                        "--steps", "31", "--skip", "10", "--preset", "config", "--order", "fast", "--quiet"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    sim = open(tmp_path / "gen_simData.csv").read()
+    assert not sim.endswith("\n")
+    lines = sim.split("\n")
+    assert lines[0] == sl.csv_sim_header(100) and len(lines) == 2 + 3
+    for row in lines[1:]:
+        pos, w, vel = sl.csv_parse_sim_row(row)
+        assert pos.shape == (100, 3) and np.all(np.isfinite(pos)) and np.all(np.isfinite(vel))
+    en = open(tmp_path / "gen_energy.csv").read().split("\n")
+    assert en[0] == "Time,PE,KE,E,p,L" and len(en) == 2 + 3
+    # the generated group is the library's seeded aggregate, centred on its centre of mass
+    g = sl.generate_bpca(100, seed=0x5EED)
+    R, m, moi = sl.csv_parse_constants(open(tmp_path / "gen_constants.csv").read(), 100)
+    assert np.allclose(R, g.R, rtol=1e-5) and np.allclose(m, g.m, rtol=1e-5)
