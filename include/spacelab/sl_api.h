@@ -183,6 +183,41 @@ typedef struct {
 int sl_profile_enable(sl_ctx* ctx, int enable, int reset);
 int sl_profile_read(sl_ctx* ctx, sl_profile* out);
 
+/* ---- run control: calibrate_dt's velocity rule ---------------------------- */
+/* calc_helpfuls (ball_group.hpp:188-195): out5 = {r_min, r_max, m_total, radius (the largest
+ * distance of a centre from the centre of mass; R[0] for one ball), soc = 4 r_max + radius}. */
+int sl_group_stats(int n, const double* pos3, const double* R, const double* m, double* out5);
+/* calc_v_collapse (ball_group.hpp:302-313): fall speed onto a group of mass m_total and
+ * radius initial_radius, integrated in steps of 0.1. */
+int sl_v_collapse(double G, double m_total, double initial_radius, double* v_collapse);
+/* getVelMax (ball_group.hpp:317-350): the largest speed among the balls moving toward the
+ * centre of mass (angle between velocity and outward radius above `cone`) within `soc` of
+ * it; with soc <= 0 the largest speed of all.  `ignored` (may be NULL): balls left out. */
+int sl_vel_max(int n, const double* pos3, const double* vel3, const double* m, double soc, double cone,
+               double* v_max, int* ignored);
+/* calibrate_dt's run length (ball_group.hpp:225-237): steps = sim_time / dt at step 0 (or
+ * when dt_old < 0), else (int)(dt_old / dt) * (steps_old - step) + step; skip = the steps per
+ * time_resolution (1 s when dt exceeds it). */
+int sl_run_length(double dt, double dt_old, int step, int steps_old, double sim_time, double time_resolution,
+                  int* steps, int* skip);
+/* The run-control state calibrate_dt keeps between calls. */
+typedef struct {
+    double kin, kout, dt;            /* springs and time step (-1 until calibrated) */
+    int steps, skip;                 /* run length and write interval */
+    double v_collapse;               /* calc_v_collapse of the group at initialisation */
+    double v_max, v_max_prev;        /* speed of the last velocity-rule call; the smallest so far */
+    double soc;                      /* sphere of consideration, from the last updateDTK */
+} sl_run_state;
+/* The state after a group is built: calc_helpfuls and calc_v_collapse of the current
+ * positions, nothing calibrated yet (the continue constructor, ball_group.hpp:92-98). */
+int sl_run_init(sl_run_state* rs, int n, const double* pos3, const double* R, const double* m, double G);
+/* calibrate_dt(step, custom_speed) (ball_group.hpp:197-238): updateDTK at custom_speed when
+ * it is positive; otherwise at max(getVelMax, v_collapse) when that is below every speed
+ * used before; then steps and skip.  `base` supplies Ha and h_min; kout = cor kin.
+ * SL_E_STATE if no calibration has happened yet. */
+int sl_calibrate_dt(sl_run_state* rs, const sl_params* base, int n, const double* pos3, const double* vel3,
+                    const double* R, const double* m, int step, double custom_speed, double cor);
+
 /* ---- initial conditions --------------------------------------------------- */
 /* Seeded ballistic particle-cluster aggregate: each new sphere is placed touching
  * a uniformly chosen earlier one in a uniformly random direction, retried until it

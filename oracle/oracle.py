@@ -171,6 +171,26 @@ def run_reference_csv(state: State, dt: float, kin: float, kout: float, n_steps:
             raise RuntimeError(f"ref_harness --csv failed ({r.returncode}): {r.stderr[-2000:]}")
 
 
+def run_reference_calib(state: State, custom_speed: float) -> np.ndarray:
+    """The reference's own calc_helpfuls, calc_v_collapse and three calibrate_dt calls
+    (ref_harness --calib): rows {kin, kout, dt, steps, skip, v_collapse, v_max,
+    v_max_prev, soc} after (step 0, velocity rule), (step 100, velocity rule with the
+    velocities halved) and (step 200, custom_speed)."""
+    if not ref_available():
+        raise FileNotFoundError(REF_HARNESS)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<4i", REF_MAGIC, state.n, 1, 1))
+            f.write(struct.pack("<3d", custom_speed, 0.0, 0.0))
+            for a in (state.pos, state.vel, state.w, state.R, state.m, state.moi):
+                f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+        r = subprocess.run([REF_HARNESS, "--calib", fin, fout], capture_output=True, text=True, cwd=td)
+        if r.returncode != 0:
+            raise RuntimeError(f"ref_harness --calib failed ({r.returncode}): {r.stderr[-2000:]}")
+        return np.frombuffer(open(fout, "rb").read(), dtype="<f8").reshape(3, 9).copy()
+
+
 def run_reference(state: State, dt: float, kin: float, kout: float, n_steps: int, skip: int):
     """Run the reference's own sim_one_step through oracle/_ref/ref_harness.
 

@@ -8,6 +8,7 @@
 //
 // Usage: ref_harness <input.bin> <output.bin>          state and energies
 //        ref_harness --csv <input.bin> <output-dir>    the reference's own CSV files
+//        ref_harness --calib <input.bin> <output.bin>  the reference's calibrate_dt
 //
 // Protocol (little-endian binary files):
 //   input : int32 {magic 0x534C4731, n, n_steps, skip}
@@ -27,6 +28,11 @@
 //      avoids reaching into the class's private section.
 //   3. for Step = 1 .. n_steps-1: sim_one_step(Step % skip == 0); energies of
 //      write steps are recorded and reset as sim_looper does (:333-344).
+//
+// Calibration mode: calc_helpfuls and calc_v_collapse of the input group, then three
+// calls of the reference's calibrate_dt: (step 0, velocity rule), (step 100, velocity rule
+// after the velocities are halved), (step 200, custom speed = the input's dt field).
+// After each: f64 {kin, kout, dt, steps, skip, v_collapse, v_max, v_max_prev, soc}.
 //
 // CSV mode: after the step-0 pass, O.sim_init_write(<output-dir>/) writes
 // constants.csv and the headers and first rows of simData.csv and energy.csv
@@ -127,12 +133,13 @@ void put_energy(std::vector<double>& out)
 int main(int argc, char** argv)
 {
     const bool csv = argc == 4 && std::strcmp(argv[1], "--csv") == 0;
-    if (argc != 3 && !csv) {
+    const bool calib = argc == 4 && std::strcmp(argv[1], "--calib") == 0;
+    if (argc != 3 && !csv && !calib) {
         std::fprintf(stderr, "usage: ref_harness <input.bin> <output.bin> | --csv <input.bin> <output-dir>\n");
         cleanup();
         return 2;
     }
-    if (csv) argv++;
+    if (csv || calib) argv++;
     FILE* fi = std::fopen(argv[1], "rb");
     if (!fi) { cleanup(); return 2; }
     int32_t hdr[4];
@@ -168,6 +175,33 @@ int main(int argc, char** argv)
     O.mom = {0, 0, 0};
     O.ang_mom = {0, 0, 0};
 
+    if (calib) {
+        std::vector<double> rec;
+        auto put = [&]() {
+            const double v[9] = {kin, kout, dt, (double)steps, (double)skip, O.v_collapse, O.v_max, O.v_max_prev, O.soc};
+            rec.insert(rec.end(), v, v + 9);
+        };
+        dt = -1;
+        kin = -1;
+        kout = -1;
+        steps = 0;
+        skip = 0;
+        O.calc_helpfuls();
+        O.calc_v_collapse();
+        O.calibrate_dt(0, 0.0);
+        put();
+        for (int i = 0; i < n; i++) O.vel[i] = O.vel[i] * 0.5;
+        O.calibrate_dt(100, 0.0);
+        put();
+        O.calibrate_dt(200, prm[0]);
+        put();
+        FILE* fo = std::fopen(argv[2], "wb");
+        if (!fo) { cleanup(); return 2; }
+        std::fwrite(rec.data(), 8, rec.size(), fo);
+        std::fclose(fo);
+        cleanup();
+        return 0;
+    }
     kin = prm[1];
     kout = prm[2];
     skip = skip_in;

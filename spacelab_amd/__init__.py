@@ -59,6 +59,13 @@ class Params(C.Structure):
         return p
 
 
+class RunState(C.Structure):
+    """sl_run_state: calibrate_dt's run-control state."""
+    _fields_ = [("kin", C.c_double), ("kout", C.c_double), ("dt", C.c_double), ("steps", C.c_int),
+                ("skip", C.c_int), ("v_collapse", C.c_double), ("v_max", C.c_double),
+                ("v_max_prev", C.c_double), ("soc", C.c_double)]
+
+
 class _Energy(C.Structure):
     _fields_ = [("PE", C.c_double), ("KE", C.c_double), ("p", C.c_double * 3),
                 ("L", C.c_double * 3), ("nonfinite", C.c_int), ("contacts", C.c_longlong)]
@@ -159,6 +166,13 @@ def lib():
                                    C.c_double, C.c_double, C.c_double] + [dp] * 7
     L.sl_collision_outcome.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, C.c_double, C.POINTER(Outcome)]
     L.sl_collision_approach.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, dp]
+    L.sl_group_stats.argtypes = [C.c_int, dp, dp, dp, dp]
+    L.sl_v_collapse.argtypes = [C.c_double, C.c_double, C.c_double, dp]
+    L.sl_vel_max.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp, ip]
+    L.sl_run_length.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, ip, ip]
+    L.sl_run_init.argtypes = [C.POINTER(RunState), C.c_int, dp, dp, dp, C.c_double]
+    L.sl_calibrate_dt.argtypes = [C.POINTER(RunState), C.POINTER(Params), C.c_int, dp, dp, dp, dp, C.c_int,
+                                  C.c_double, C.c_double]
     cp, ll = C.c_char_p, C.c_longlong
     for name, args in (("sl_csv_sim_header", [C.c_int, cp, ll]),
                        ("sl_csv_sim_row", [C.c_int, dp, dp, dp, cp, ll]),
@@ -212,6 +226,31 @@ def preset(name: str = "reference") -> Params:
     if rc:
         raise SpacelabError(rc, "sl_params_preset")
     return p
+
+
+class RunControl:
+    """calibrate_dt's run control (ball_group.hpp:197-238): built from a group as the
+    continue constructor does (calc_helpfuls, calc_v_collapse), then calibrate(step,
+    custom_speed) with the velocity rule (custom_speed <= 0) or a custom speed."""
+
+    def __init__(self, g: BallGroup, base: Params | None = None, cor: float = 0.4):
+        n = g.num_particles
+        self.base = (base or preset("reference")).copy()
+        self.cor = cor
+        self.state = RunState()
+        rc = lib().sl_run_init(C.byref(self.state), n, _dp(_f64(g.pos, (n, 3))), _dp(_f64(g.R, (n,))),
+                               _dp(_f64(g.m, (n,))), self.base.G)
+        if rc:
+            raise SpacelabError(rc, "sl_run_init")
+
+    def calibrate(self, g: BallGroup, step: int, custom_speed: float = 0.0) -> RunState:
+        n = g.num_particles
+        rc = lib().sl_calibrate_dt(C.byref(self.state), C.byref(self.base), n, _dp(_f64(g.pos, (n, 3))),
+                                   _dp(_f64(g.vel, (n, 3))), _dp(_f64(g.R, (n,))), _dp(_f64(g.m, (n,))),
+                                   int(step), float(custom_speed), self.cor)
+        if rc:
+            raise SpacelabError(rc, "sl_calibrate_dt")
+        return self.state
 
 
 def calibrate(p: Params, R: np.ndarray, m: np.ndarray, velocity: float, cor: float,

@@ -157,25 +157,27 @@ public:
     }
 };
 
-// kin, kout and dt by updateDTK at a speed (sl_calibrate's reference rule), and the
-// step count and write interval of calibrate_dt's first call.
+// calibrate_dt's run control (sl_run_init / sl_calibrate_dt): built from a group as the
+// reference's constructors do (calc_helpfuls, calc_v_collapse), then calibrate(step,
+// custom_speed): updateDTK at custom_speed when it is positive, else at the largest
+// speed of the balls falling toward the centre of mass (at least the collapse speed)
+// when that is below every speed used before; steps and skip follow.
 struct RunControl {
-    double kin = 0, kout = 0, dt = 0;
-    int steps = 0, skip = 0;
+    sl_run_state rs{};
+    sl_params base{};
+    double cor = 0.4;   // dust_const.hpp:27
 
-    static RunControl calibrate(const Ball_group& g, double speed, double cor, int dt_rule = 0)
+    RunControl(const Ball_group& g, const sl_params& base_params, double cor_ = 0.4) : base(base_params), cor(cor_)
     {
-        sl_params p;
-        sl_params_preset(&p, 0);
-        const int rc = sl_calibrate(&p, g.num_particles, g.R.data(), g.m.data(), speed, cor, dt_rule);
-        if (rc) throw Error(rc, "sl_calibrate");
-        RunControl r;
-        r.kin = p.kin;
-        r.kout = p.kout;
-        r.dt = p.dt;
-        r.steps = (int)(kSimTimeSeconds / r.dt);
-        r.skip = kTimeResolution / r.dt > 1. ? (int)std::floor(kTimeResolution / r.dt) : (int)std::floor(1. / r.dt);
-        return r;
+        const int rc = sl_run_init(&rs, g.num_particles, &g.pos[0].x, g.R.data(), g.m.data(), base.G);
+        if (rc) throw Error(rc, "sl_run_init");
+    }
+
+    void calibrate(const Ball_group& g, int step, double custom_speed)
+    {
+        const int rc = sl_calibrate_dt(&rs, &base, g.num_particles, &g.pos[0].x, &g.vel[0].x, g.R.data(), g.m.data(),
+                                       step, custom_speed, cor);
+        if (rc) throw Error(rc, "sl_calibrate_dt");
     }
 };
 
@@ -185,7 +187,7 @@ public:
     double simTimeElapsed = 0;
     bool progress = true;
 
-    Collider(Ball_group& g, const sl_params& p) : O(g)
+    Collider(Ball_group& g, const sl_params& p) : O(g), params(p)
     {
         int rc = sl_create(&ctx, g.num_particles, &p);
         if (rc) throw Error(rc, rc == SL_E_NOGPU ? "no usable gfx950 GPU" : "sl_create");
@@ -193,6 +195,16 @@ public:
                        g.R.data(), g.m.data(), g.moi.data(), nullptr);
         check(rc, "sl_upload");
         dt = p.dt;
+    }
+
+    // New springs and time step for the following steps (the state stays on the GPU).
+    void set_run(double kin, double kout, double dt_new)
+    {
+        params.kin = kin;
+        params.kout = kout;
+        params.dt = dt_new;
+        check(sl_set_params(ctx, &params), "sl_set_params");
+        dt = dt_new;
     }
     ~Collider()
     {
@@ -244,13 +256,17 @@ public:
 
     // sim_looper (ColliderSingleCore.cpp:291-384): Step = 1 .. steps-1, write steps
     // every `skip`, an energy row at 12 digits after each, the buffers flushed to the
-    // files at the first write and at every tenth (or after 30 minutes).
-    void sim_looper(int steps, int skip)
+    // files at the first write and at every tenth (or after 30 minutes).  With `dynamic`
+    // (the reference's dynamicTime, off in dust_const.hpp:10) each write step ends with
+    // calibrate_dt(Step) by the velocity rule; new springs, dt, run length and write
+    // interval take effect from the next step.
+    void sim_looper(int steps, int skip, RunControl* dynamic = nullptr)
     {
         if (skip < 1) throw Error(SL_E_ARG, "skip must be at least 1");
         const time_t t0 = std::time(nullptr);
         int pending = 0;   // non-write steps not yet run
-        for (int Step = 1; Step < steps; Step++) {
+        if (dynamic) dynamic->rs.steps = steps;
+        for (int Step = 1; Step < (dynamic ? dynamic->rs.steps : steps); Step++) {
             if (Step % skip != 0) {
                 pending++;
                 continue;
@@ -265,6 +281,12 @@ public:
             energyBuffer += energy_row(SL_CSV_ENERGY_DIGITS);
             reset_energy();
             if (std::time(nullptr) - lastWrite > 1800 || Step / skip % 10 == 0) flush();
+            if (dynamic) {
+                const double dt_before = dynamic->rs.dt;
+                dynamic->calibrate(O, Step, 0.0);
+                if (dynamic->rs.dt != dt_before) set_run(dynamic->rs.kin, dynamic->rs.kout, dynamic->rs.dt);
+                skip = dynamic->rs.skip;
+            }
         }
         if (pending) steps_then_write(pending, false);
     }
@@ -289,6 +311,7 @@ public:
 
 private:
     Ball_group& O;
+    sl_params params;
     sl_ctx* ctx = nullptr;
     double dt = 0;
     std::string ballBuffer, energyBuffer;

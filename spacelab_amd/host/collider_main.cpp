@@ -2,15 +2,19 @@
 //
 // Builds the Ball_group (continued from a run's CSV files, as
 // Ball_group(path, filename, customVel) does, ball_group.hpp:92-98, or generated),
-// centres it on its centre of mass, calibrates kin, kout and dt at a custom speed
-// (calibrate_dt -> updateDTK, ball_group.hpp:197-238,1313-1329) unless they are given,
-// runs init_conditions, sim_init_write and sim_looper (ColliderSingleCore.cpp:291-384)
-// and writes <out>constants.csv, <out>simData.csv and <out>energy.csv.
+// calibrates kin, kout, dt, steps and skip as calibrate_dt does (ball_group.hpp:197-238:
+// at --v-custom when it is positive, else by the velocity rule; a continued run defaults
+// to the velocity rule like the reference's continue constructor, a generated group to
+// dust_const.hpp's v_custom), centres it on its centre of mass, runs init_conditions,
+// sim_init_write and sim_looper (ColliderSingleCore.cpp:291-384) and writes
+// <out>constants.csv, <out>simData.csv and <out>energy.csv.  Explicit --dt/--kin/--kout
+// and --steps/--skip override the calibration; --dynamic-time recalibrates at write
+// steps (the reference's dynamicTime).
 //
 //   spacelab_collider (--continue PATH PREFIX | --generate N SEED) [--out PREFIX]
 //       [--v-custom V] [--cor C] [--dt DT --kin KIN --kout KOUT] [--steps N] [--skip K]
 //       [--order ordered|fast] [--precision fp64|mixed] [--preset reference|config]
-//       [--no-center] [--quiet]
+//       [--no-center] [--dynamic-time] [--quiet]
 #include "collider.hpp"
 
 #include <cstdlib>
@@ -25,7 +29,7 @@ namespace {
                          "usage: spacelab_collider (--continue PATH PREFIX | --generate N SEED) [--out PREFIX]\n"
                          "  [--v-custom V] [--cor C] [--dt DT --kin KIN --kout KOUT] [--steps N] [--skip K]\n"
                          "  [--order ordered|fast] [--precision fp64|mixed] [--preset reference|config]\n"
-                         "  [--no-center] [--quiet]\n",
+                         "  [--no-center] [--dynamic-time] [--quiet]\n",
                  msg);
     std::exit(2);
 }
@@ -38,11 +42,11 @@ int main(int argc, char** argv)
     std::string path, prefix, out = "Unnamed_";   // output_prefix default, dust_const.hpp:67
     int gen_n = 0;
     unsigned long long gen_seed = 0;
-    double v_custom = 0.36301555459799423;        // dust_const.hpp:34
+    double v_custom = -1;                         // unset: chosen by the mode below
     double cor = 0.4;                             // dust_const.hpp:27
     double dt = 0, kin = 0, kout = 0;
     int steps = -1, skip = -1, order = SL_ORDER_ORDERED, precision = SL_PREC_FP64, preset = 0;
-    bool center = true, quiet = false, cont = false;
+    bool center = true, quiet = false, cont = false, dynamic = false;
     for (int i = 1; i < argc; i++) {
         const std::string a = argv[i];
         auto next = [&]() -> const char* {
@@ -77,40 +81,45 @@ int main(int argc, char** argv)
             preset = v == "config" ? 1 : v == "reference" ? 0 : -1;
             if (preset < 0) usage("--preset is reference or config");
         } else if (a == "--no-center") center = false;
+        else if (a == "--dynamic-time") dynamic = true;
         else if (a == "--quiet") quiet = true;
         else usage(("unknown argument " + a).c_str());
     }
     if (cont == (gen_n > 0)) usage("give exactly one of --continue and --generate");
     try {
         Ball_group O = cont ? Ball_group::from_csv(path, prefix) : Ball_group::generate(gen_n, gen_seed);
-        if (center) O.to_origin();
-        RunControl rc = RunControl::calibrate(O, v_custom, cor);
-        if (dt > 0) rc.dt = dt;
-        if (kin > 0) rc.kin = kin;
-        if (kout > 0) rc.kout = kout;
-        if (steps >= 0) rc.steps = steps;
-        if (skip > 0) rc.skip = skip;
         sl_params p;
         sl_params_preset(&p, preset);
-        p.kin = rc.kin;
-        p.kout = rc.kout;
-        p.dt = rc.dt;
+        // the continue constructor calibrates with customVel 0 (the velocity rule), the
+        // generator at v_custom; both before centring (ball_group.hpp:67-98)
+        if (v_custom < 0) v_custom = cont ? 0.0 : 0.36301555459799423;   // dust_const.hpp:34
+        RunControl rc(O, p, cor);
+        if (!(dt > 0 && kin > 0 && kout > 0 && steps >= 0 && skip > 0)) rc.calibrate(O, 0, v_custom);
+        if (center) O.to_origin();
+        if (dt > 0) rc.rs.dt = dt;
+        if (kin > 0) rc.rs.kin = kin;
+        if (kout > 0) rc.rs.kout = kout;
+        if (steps >= 0) rc.rs.steps = steps;
+        if (skip > 0) rc.rs.skip = skip;
+        p.kin = rc.rs.kin;
+        p.kout = rc.rs.kout;
+        p.dt = rc.rs.dt;
         p.order = order;
         p.precision = precision;
         if (!quiet)
-            std::cerr << "==================\ndt: " << p.dt << "\nk: " << p.kin << "\nSkip: " << rc.skip
-                      << "\nSteps: " << rc.steps << "\n==================\n";
+            std::cerr << "==================\ndt: " << p.dt << "\nk: " << p.kin << "\nSkip: " << rc.rs.skip
+                      << "\nSteps: " << rc.rs.steps << "\n==================\n";
         Collider c(O, p);
         c.progress = !quiet;
         c.init_conditions();
         c.sim_init_write(out);
         const time_t t0 = std::time(nullptr);
-        c.sim_looper(rc.steps, rc.skip);
+        c.sim_looper(rc.rs.steps, rc.rs.skip, dynamic ? &rc : nullptr);
         c.flush();
         if (!quiet)
             std::cerr << "Simulation complete!\n"
-                      << O.num_particles << " Particles and " << rc.steps << " Steps.\n"
-                      << "Simulated time: " << rc.steps * p.dt << " seconds\n"
+                      << O.num_particles << " Particles and " << rc.rs.steps << " Steps.\n"
+                      << "Simulated time: " << rc.rs.steps * p.dt << " seconds\n"
                       << "Computation time: " << std::time(nullptr) - t0 << " seconds\n"
                       << "Output: " << c.output_prefix << "{constants,simData,energy}.csv\n";
     } catch (const Error& e) {

@@ -171,11 +171,37 @@ def make_csv_case():
     print(f"{CSV_CASE}: reference CSV files and {os.path.basename(fname)}")
 
 
+def make_calib_cases():
+    """The reference's own calibrate_dt (ref_harness --calib) on four groups: two touching
+    clusters, the 1000-sphere seeded aggregate, and a resting cluster (speed below the
+    collapse speed).  Stored in tests/golden/calib/."""
+    d = os.path.join(OUT, "calib")
+    os.makedirs(d, exist_ok=True)
+    import spacelab_amd as sl
+    cases = {}
+    for name, n, seed in (("cluster16", 16, 1601), ("cluster64", 64, 6401)):
+        pos, vel, w, R, m, _, _, _ = case_cluster(n, seed)
+        cases[name] = (pos, vel, w, R, m)
+    g = sl.generate_bpca(1000, seed=0x5EED0000, v_sigma=1e-3)
+    cases["bpca1000"] = (g.pos, g.vel, g.w, g.R, g.m)
+    pos, vel, w, R, m, _, _, _ = case_cluster(16, 1603)
+    cases["resting16"] = (pos, np.zeros_like(vel), w, R, m)
+    for name, (pos, vel, w, R, m) in cases.items():
+        mo = np.array([moi(r, mm) for r, mm in zip(R, m)])
+        st = oracle.State(pos, vel, w, R, m, mo)
+        custom = 2.5
+        rec = oracle.run_reference_calib(st, custom)
+        np.savez(os.path.join(d, name + ".npz"), pos=st.pos, vel=st.vel, R=st.R, m=st.m,
+                 custom_speed=np.array(custom), records=rec)
+        print(f"calib/{name}: kin {rec[0, 0]:.6g} steps {rec[0, 3]:.0f} skip {rec[0, 4]:.0f}")
+
+
 def main():
     if not oracle.ref_available():
         oracle.build()
     if not oracle.ref_available():
         raise SystemExit("reference harness not built (reference sources absent)")
+    make_calib_cases()
     make_csv_case()
     for name, (fn, runs) in CASES.items():
         pos, vel, w, R, m, kin, kout, dt = fn()

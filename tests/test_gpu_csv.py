@@ -110,3 +110,19 @@ def test_driver_generate_mode(tmp_path):
     g = sl.generate_bpca(100, seed=0x5EED)
     R, m, moi = sl.csv_parse_constants(open(tmp_path / "gen_constants.csv").read(), 100)
     assert np.allclose(R, g.R, rtol=1e-5) and np.allclose(m, g.m, rtol=1e-5)
+
+
+def test_driver_dynamic_time(tmp_path):
+    """--dynamic-time recalibrates at write steps by the velocity rule.  A generated
+    aggregate starts calibrated at v_custom (0.36 cm/s); at the first write step the rule
+    takes its balls' largest infall speed (about 3e-3 cm/s), dt grows, and calibrate_dt's
+    integer rescaling (int)(dt_old / dt) = 0 ends the run at that step, as the reference's
+    loop would (the same calls on the CPU give steps = 10)."""
+    r = subprocess.run([sl.collider_program(), "--generate", "50", "0x5EED", "--out", str(tmp_path) + "/dyn_",
+                        "--steps", "41", "--skip", "10", "--dynamic-time", "--order", "ordered", "--quiet"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    en = open(tmp_path / "dyn_energy.csv").read().split("\n")
+    assert len(en) == 3                      # header, initial row, the row of step 10
+    sim = open(tmp_path / "dyn_simData.csv").read().split("\n")
+    assert len(sim) == 3
