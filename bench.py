@@ -158,11 +158,23 @@ def cpu_omp(g, p, threads, target_s=8.0):
             "kind": "port", "sample": f"ordered gather, OpenMP {threads} threads, {k} pair passes, {sec:.1f} s"}
 
 
+def host_info():
+    model = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
 def energy_scale(e):
     return abs(e.PE) + e.KE
 
 
-def ensemble(n_coll=512, steps=200, warmup=20, horizon=50000, seed=3):
+def ensemble(n_coll=512, steps=200, warmup=20, horizon=50000, seed=0x5EED0003):
     """Batched collision ensemble at full size: target N log-uniform 200-4000,
     projectile N log-uniform 1-500, speed uniform 10-1000 cm/s, b uniform in
     [0, 1] (R_t + R_p), springs and dt calibrated per collision at its impact speed
@@ -211,7 +223,29 @@ def ensemble(n_coll=512, steps=200, warmup=20, horizon=50000, seed=3):
                     "are a bounded sample of the 50k-step run, inputs resident in HBM"}
 
 
-def collision(steps=50000, seed=7, n_t=2000, n_p=200):
+def end_to_end(n=5000, steps=20001, skip=1000, seed=SEED):
+    """The C++ driver (spacelab_collider) on the seeded aggregate: sim_looper with write
+    steps every `skip` (state download, simData and energy rows, file output), fast FP64.
+    Wall time of the loop and its final flush, as the driver measures it."""
+    import subprocess
+    import tempfile
+    import spacelab_amd as sl
+    with tempfile.TemporaryDirectory() as td:
+        r = subprocess.run([sl.collider_program(), "--generate", str(n), hex(seed), "--out", td + "/e2e_",
+                            "--steps", str(steps), "--skip", str(skip), "--preset", "config", "--order", "fast",
+                            "--quiet", "--timing"], capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"spacelab_collider failed: {r.stderr[-2000:]}")
+        f = r.stdout.split()
+        sec, st = float(f[f.index("loop_seconds") + 1]), int(f[f.index("steps") + 1])
+        size = sum(os.path.getsize(os.path.join(td, x)) for x in os.listdir(td))
+    pairs = n * (n - 1) // 2
+    return {"n_spheres": n, "steps": st - 1, "write_every": skip, "seconds": sec,
+            "pairs_per_s": (st - 1) * pairs / sec, "ms_per_step": 1e3 * sec / (st - 1), "csv_bytes": size,
+            "note": "spacelab_collider --generate: sim_looper with write steps, downloads and CSV files"}
+
+
+def collision(steps=50000, seed=0x5EED0002, n_t=2000, n_p=200):
     """One aggregate collision run in full: seeded BPCA target (2000 spheres) and projectile
     (200, independent seed), random orientations, impact speed uniform in 10-1000 cm/s and
     impact parameter uniform in [0, 1] (R_t + R_p) (drawn from ``seed``), the projectile
@@ -303,11 +337,12 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=500)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["ensemble", "surrogate", "mlp", "collision"], default=None,
+    ap.add_argument("--only", choices=["ensemble", "surrogate", "mlp", "collision", "e2e"], default=None,
                     help="run only this part and print its object (for profiling)")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-collision", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run the plain copy kernel on this many MiB first (counter calibration)")
@@ -320,7 +355,7 @@ def main():
     import spacelab_amd as sl
     if a.only:
         out = {"ensemble": lambda: ensemble(steps=a.ensemble_steps), "surrogate": surrogate_eval,
-               "mlp": mlp_only, "collision": collision}[a.only]()
+               "mlp": mlp_only, "collision": collision, "e2e": end_to_end}[a.only]()
         print(json.dumps({a.only: out}), flush=True)
         return
     if a.copy_probe_mb > 0:
@@ -423,6 +458,10 @@ def main():
         log(f"cpu baseline: oracle ordered gather on {threads} threads")
         extra["cpu_omp"] = cpu_omp(g, p, threads)
 
+    if not a.no_e2e:
+        log("end to end: spacelab_collider, 5000 spheres, 20000 steps, write every 1000")
+        extra["end_to_end"] = end_to_end()
+        log(f"end to end: {extra['end_to_end']['pairs_per_s']:.4e} pairs/s")
     if not a.no_collision:
         log("collision: 2000 + 200 spheres, 50000 steps, fp64 and mixed")
         extra["collision"] = collision()
@@ -449,6 +488,8 @@ def main():
                        "n_spheres": n, "pairs_per_step": pairs, "mode": primary,
                        "parallelism": "single GPU"},
             "modes": modes_out, "roofline": roof, "cpu_baseline": cpu}
+    if cpu is not None:
+        cpu["host"] = host_info()
     line.update(extra)
     print(json.dumps(line), flush=True)
 

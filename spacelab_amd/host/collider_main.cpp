@@ -14,9 +14,13 @@
 //   spacelab_collider (--continue PATH PREFIX | --generate N SEED) [--out PREFIX]
 //       [--v-custom V] [--cor C] [--dt DT --kin KIN --kout KOUT] [--steps N] [--skip K]
 //       [--order ordered|fast] [--precision fp64|mixed] [--preset reference|config]
-//       [--no-center] [--dynamic-time] [--quiet]
+//       [--no-center] [--dynamic-time] [--timing] [--quiet]
+//
+// --timing prints the wall time of sim_looper and the final flush (write steps, state
+// downloads and CSV output included) on stdout.
 #include "collider.hpp"
 
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
@@ -29,7 +33,7 @@ namespace {
                          "usage: spacelab_collider (--continue PATH PREFIX | --generate N SEED) [--out PREFIX]\n"
                          "  [--v-custom V] [--cor C] [--dt DT --kin KIN --kout KOUT] [--steps N] [--skip K]\n"
                          "  [--order ordered|fast] [--precision fp64|mixed] [--preset reference|config]\n"
-                         "  [--no-center] [--dynamic-time] [--quiet]\n",
+                         "  [--no-center] [--dynamic-time] [--timing] [--quiet]\n",
                  msg);
     std::exit(2);
 }
@@ -46,7 +50,7 @@ int main(int argc, char** argv)
     double cor = 0.4;                             // dust_const.hpp:27
     double dt = 0, kin = 0, kout = 0;
     int steps = -1, skip = -1, order = SL_ORDER_ORDERED, precision = SL_PREC_FP64, preset = 0;
-    bool center = true, quiet = false, cont = false, dynamic = false;
+    bool center = true, quiet = false, cont = false, dynamic = false, timing = false;
     for (int i = 1; i < argc; i++) {
         const std::string a = argv[i];
         auto next = [&]() -> const char* {
@@ -82,6 +86,7 @@ int main(int argc, char** argv)
             if (preset < 0) usage("--preset is reference or config");
         } else if (a == "--no-center") center = false;
         else if (a == "--dynamic-time") dynamic = true;
+        else if (a == "--timing") timing = true;
         else if (a == "--quiet") quiet = true;
         else usage(("unknown argument " + a).c_str());
     }
@@ -114,8 +119,11 @@ int main(int argc, char** argv)
         c.init_conditions();
         c.sim_init_write(out);
         const time_t t0 = std::time(nullptr);
+        const auto c0 = std::chrono::steady_clock::now();
         c.sim_looper(rc.rs.steps, rc.rs.skip, dynamic ? &rc : nullptr);
         c.flush();
+        const double loop_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
+        if (timing) std::printf("loop_seconds %.6f steps %d balls %d\n", loop_s, rc.rs.steps, O.num_particles);
         if (!quiet)
             std::cerr << "Simulation complete!\n"
                       << O.num_particles << " Particles and " << rc.rs.steps << " Steps.\n"

@@ -75,6 +75,27 @@ def test_graph_segments_across_calls():
     assert [_ebytes(e) for e in ew] == [_ebytes(e) for e in ew1]
 
 
+def test_graph_cache_eviction():
+    """More distinct segment lengths than the graph cache holds (16): the cache is
+    emptied while earlier segments may still run, and the state equals a run with graphs
+    off, bit for bit."""
+    g = sl.generate_bpca(300, seed=0x5EED0103, v_sigma=2.0)
+    p = _params(g)
+    lengths = list(range(3, 43, 2))          # 20 lengths, each a graph of its own
+    outs = []
+    for graphs in (1, 0):
+        with sl.Collider(g.num_particles, p) as c:
+            c.set_option("tile_split", 2)
+            c.set_option("graphs", graphs)
+            c.upload(g)
+            c.init_conditions()
+            for n in lengths + lengths[::-1]:
+                c.step(n, 0)
+            outs.append(c.download())
+    for k in FIELDS:
+        assert bits_equal(getattr(outs[0], k), getattr(outs[1], k)), k
+
+
 def _ensemble(sizes, seed0):
     groups = [sl.generate_bpca(n, seed=seed0 + k, v_sigma=1.0 + k) for k, n in enumerate(sizes)]
     params = [_params(g, velocity=V_CUSTOM * (1 + 0.5 * k)) for k, g in enumerate(groups)]
