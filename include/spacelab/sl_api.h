@@ -217,6 +217,9 @@ int sl_run_init(sl_run_state* rs, int n, const double* pos3, const double* R, co
  * SL_E_STATE if no calibration has happened yet. */
 int sl_calibrate_dt(sl_run_state* rs, const sl_params* base, int n, const double* pos3, const double* vel3,
                     const double* R, const double* m, int step, double custom_speed, double cor);
+/* to_origin (ball_group.hpp:542-547): subtracts the centre of mass (getCOM, :519-530) from
+ * every position, in place.  SL_E_ARG when the total mass is not positive. */
+int sl_to_origin(int n, double* pos3, const double* m);
 
 /* ---- initial conditions --------------------------------------------------- */
 /* Seeded ballistic particle-cluster aggregate: each new sphere is placed touching
@@ -230,6 +233,21 @@ int sl_gen_bpca(int n, unsigned long long seed, double r_med, double sigma, doub
                 double* moi);
 
 /* ---- collisions ------------------------------------------------------------- */
+/* The reference's two-cluster set-up of two loaded groups (sim_init_two_cluster,
+ * ball_group.hpp:1368-1442; the collision constructor, :100-112): the projectile (n_p
+ * balls) is moved by (its radius + the target's radius + 2 r_max of the target) along
+ * (cos a, sin a, 0) for the impact angle a (dust_const.hpp:36 impactParameter), given
+ * velocity -v_impact along x (dust_const.hpp:34 v_custom) and the target the balancing
+ * +(M_p/M_t) v_impact; the output arrays hold the target, then the projectile (n_t + n_p
+ * entries).  Radii are the largest centre distance from each group's centre of mass.
+ * v2 (may be NULL) receives {projectile speed, target speed} along x.  Centring and
+ * calibration follow separately, as in the constructor: sl_run_init, sl_calibrate_dt,
+ * then sl_to_origin. */
+int sl_two_cluster(int n_t, const double* t_pos3, const double* t_vel3, const double* t_w3, const double* t_R,
+                   const double* t_m, const double* t_moi, int n_p, const double* p_pos3, const double* p_vel3,
+                   const double* p_w3, const double* p_R, const double* p_m, const double* p_moi, double v_impact,
+                   double impact_angle, double* pos3, double* vel3, double* w3, double* R, double* m, double* moi,
+                   double* v2);
 /* Two-aggregate collision (the two-cluster set-up of ball_group.hpp:1368-1442, with
  * the impact parameter as a distance, no reference counterpart for the rest): a
  * target of n_t spheres and a projectile of n_p spheres, each a seeded aggregate as

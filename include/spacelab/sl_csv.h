@@ -17,6 +17,10 @@
  * energy row, written by sim_init_write, has 6).  |w|, |p| and |L| are
  * sqrt(x*x + y*y + z*z) evaluated left to right (vec3.hpp:126).
  *
+ * A continued run reads <prefix>constants.csv, or <prefix>Constants.csv where only that
+ * exists (the Generator's files, Generator.cpp:168-170, whose energy layout
+ * SL_CSV_ENERGY_HEADER_BOUND adds Bound,Unbound,m).
+ *
  * Formatting functions follow snprintf: they return the length of the full text
  * (without the terminating NUL) and write at most cap bytes including the NUL, so a
  * call with buf = NULL, cap = 0 sizes the buffer.  Host code only: no GPU is used. */
@@ -29,6 +33,8 @@ extern "C" {
 
 #define SL_CSV_PROPERTIES 11   /* columns per ball in simData.csv (dust_const.hpp:45) */
 #define SL_CSV_ENERGY_HEADER "Time,PE,KE,E,p,L"
+/* the Generator's energy layout (Generator.cpp:195), read but never written */
+#define SL_CSV_ENERGY_HEADER_BOUND "Time,PE,KE,E,p,L,Bound,Unbound,m"
 #define SL_CSV_STATE_DIGITS 6
 #define SL_CSV_ENERGY_DIGITS 12
 
@@ -51,9 +57,14 @@ int sl_csv_count_balls(const char* row);
 int sl_csv_parse_sim_row(const char* row, int n, double* pos3, double* w3, double* vel3);
 /* R, m, moi of n balls from constants.csv text (loadConsts, ball_group.hpp:1018-1040). */
 int sl_csv_parse_constants(const char* text, int n, double* R, double* m, double* moi);
+/* Columns of an energy.csv file from its header line: 6 for SL_CSV_ENERGY_HEADER, 9 for
+ * SL_CSV_ENERGY_HEADER_BOUND (the Generator's, Generator.cpp:195); SL_E_ARG otherwise. */
+int sl_csv_energy_columns(const char* header);
+/* The ncols (6 or 9) numbers of one energy.csv row, which must hold exactly that many. */
+int sl_csv_parse_energy_row(const char* row, int ncols, double* out);
 /* The last line of a file as getLastLine reads it (ball_group.hpp:1043-1077): the text
- * after the final '\n' (empty if the file ends with one), or the whole file if it has
- * none.  Returns its length (snprintf rules), SL_E_ARG if the file cannot be read. */
+ * after the final '\n' (empty if the file ends with one, or if that '\n' is its first
+ * byte), or the whole file if it has none.  Returns its length (snprintf rules), SL_E_ARG if the file cannot be read. */
 long long sl_csv_last_line(const char* path, char* buf, long long cap);
 
 #ifdef __cplusplus

@@ -191,6 +191,36 @@ def run_reference_calib(state: State, custom_speed: float) -> np.ndarray:
         return np.frombuffer(open(fout, "rb").read(), dtype="<f8").reshape(3, 9).copy()
 
 
+TWO_CLUSTER_FIELDS = ("kin", "kout", "dt", "steps", "skip", "v_collapse", "v_max", "v_max_prev", "soc",
+                      "m_total", "initial_radius", "r_max")
+
+
+def run_reference_two_cluster(directory: str, projectile: str, target: str, custom_speed: float) -> dict:
+    """The reference's own two-cluster constructor, Ball_group(path, projectile, target,
+    customVel) (ball_group.hpp:100-112, 1368-1442), on the run files
+    <directory>/<name>simData.csv and constants.csv (ref_harness --two-cluster).
+    Returns the merged group after calibrate_dt and to_origin (pos, vel, w, R, m, moi)
+    and its run-control values (TWO_CLUSTER_FIELDS)."""
+    if not ref_available():
+        raise FileNotFoundError(REF_HARNESS)
+    directory = os.path.abspath(directory) + os.sep
+    with tempfile.TemporaryDirectory() as td:
+        fout = os.path.join(td, "out.bin")
+        r = subprocess.run([REF_HARNESS, "--two-cluster", directory, projectile, target, repr(float(custom_speed)),
+                            fout], capture_output=True, text=True, cwd=td)
+        if r.returncode != 0:
+            raise RuntimeError(f"ref_harness --two-cluster failed ({r.returncode}): {r.stderr[-2000:]}")
+        raw = open(fout, "rb").read()
+    n = struct.unpack("<i", raw[:4])[0]
+    v = np.frombuffer(raw[4:], dtype="<f8")
+    out, k = {}, 0
+    for name, size in (("pos", 3), ("vel", 3), ("w", 3), ("R", 1), ("m", 1), ("moi", 1)):
+        out[name] = v[k:k + size * n].reshape((n, 3) if size == 3 else (n,)).copy()
+        k += size * n
+    out["run"] = v[k:k + len(TWO_CLUSTER_FIELDS)].copy()
+    return out
+
+
 def run_reference(state: State, dt: float, kin: float, kout: float, n_steps: int, skip: int):
     """Run the reference's own sim_one_step through oracle/_ref/ref_harness.
 

@@ -9,6 +9,8 @@
 // Usage: ref_harness <input.bin> <output.bin>          state and energies
 //        ref_harness --csv <input.bin> <output-dir>    the reference's own CSV files
 //        ref_harness --calib <input.bin> <output.bin>  the reference's calibrate_dt
+//        ref_harness --two-cluster <dir/> <projectile> <target> <custom> <output.bin>
+//                                                      the reference's two-cluster constructor
 //
 // Protocol (little-endian binary files):
 //   input : int32 {magic 0x534C4731, n, n_steps, skip}
@@ -33,6 +35,15 @@
 // calls of the reference's calibrate_dt: (step 0, velocity rule), (step 100, velocity rule
 // after the velocities are halved), (step 200, custom speed = the input's dt field).
 // After each: f64 {kin, kout, dt, steps, skip, v_collapse, v_max, v_max_prev, soc}.
+//
+// Two-cluster mode: Ball_group(dir, projectile, target, custom) -- the reference's own
+// constructor (ball_group.hpp:100-112): sim_init_two_cluster loads both groups from
+// <dir><name>simData.csv and constants.csv, offsets, kicks and merges them (:1368-1442),
+// then calc_v_collapse, calibrate_dt(0, custom) and simInit_cond_and_center (to_origin,
+// init_conditions).  Output: int32 n; f64 pos[3n], vel[3n], w[3n], R[n], m[n], moi[n];
+// f64 {kin, kout, dt, steps, skip, v_collapse, v_max, v_max_prev, soc, m_total,
+// initial_radius, r_max}.  init_conditions reads the uninitialised distances array
+// (ball_group.hpp:741,796), so its forces and energies are not part of the output.
 //
 // CSV mode: after the step-0 pass, O.sim_init_write(<output-dir>/) writes
 // constants.csv and the headers and first rows of simData.csv and energy.csv
@@ -130,8 +141,33 @@ void put_energy(std::vector<double>& out)
 }
 }  // namespace
 
+int two_cluster(char** argv)
+{
+    const Ball_group g(argv[2], argv[3], argv[4], std::atof(argv[5]));
+    FILE* fo = std::fopen(argv[6], "wb");
+    if (!fo) return 2;
+    const int32_t n = g.num_particles;
+    std::fwrite(&n, 4, 1, fo);
+    for (const vec3* a : {g.pos, g.vel, g.w})
+        for (int i = 0; i < n; i++) {
+            const double v[3] = {a[i].x, a[i].y, a[i].z};
+            std::fwrite(v, 8, 3, fo);
+        }
+    for (const double* a : {g.R, g.m, g.moi}) std::fwrite(a, 8, n, fo);
+    const double tail[12] = {kin, kout, dt, (double)steps, (double)skip, g.v_collapse, g.v_max,
+                             g.v_max_prev, g.soc, g.m_total, g.initial_radius, g.r_max};
+    std::fwrite(tail, 8, 12, fo);
+    std::fclose(fo);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
+    if (argc == 7 && std::strcmp(argv[1], "--two-cluster") == 0) {
+        const int rc = two_cluster(argv);
+        cleanup();
+        return rc;
+    }
     const bool csv = argc == 4 && std::strcmp(argv[1], "--csv") == 0;
     const bool calib = argc == 4 && std::strcmp(argv[1], "--calib") == 0;
     if (argc != 3 && !csv && !calib) {

@@ -173,6 +173,8 @@ def lib():
     L.sl_run_init.argtypes = [C.POINTER(RunState), C.c_int, dp, dp, dp, C.c_double]
     L.sl_calibrate_dt.argtypes = [C.POINTER(RunState), C.POINTER(Params), C.c_int, dp, dp, dp, dp, C.c_int,
                                   C.c_double, C.c_double]
+    L.sl_to_origin.argtypes = [C.c_int, dp, dp]
+    L.sl_two_cluster.argtypes = [C.c_int] + [dp] * 6 + [C.c_int] + [dp] * 6 + [C.c_double, C.c_double] + [dp] * 7
     cp, ll = C.c_char_p, C.c_longlong
     for name, args in (("sl_csv_sim_header", [C.c_int, cp, ll]),
                        ("sl_csv_sim_row", [C.c_int, dp, dp, dp, cp, ll]),
@@ -184,6 +186,8 @@ def lib():
     L.sl_csv_count_balls.argtypes = [cp]
     L.sl_csv_parse_sim_row.argtypes = [cp, C.c_int, dp, dp, dp]
     L.sl_csv_parse_constants.argtypes = [cp, C.c_int, dp, dp, dp]
+    L.sl_csv_energy_columns.argtypes = [cp]
+    L.sl_csv_parse_energy_row.argtypes = [cp, C.c_int, dp]
     L.sl_collision_descriptors.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_double,
                                            C.c_double, dp]
     fp = C.POINTER(C.c_float)
@@ -288,6 +292,50 @@ class BallGroup:
         cp = lambda a: None if a is None else np.array(a, dtype=np.float64, copy=True)
         return BallGroup(cp(self.pos), cp(self.vel), cp(self.w), cp(self.R), cp(self.m),
                          cp(self.moi), cp(self.acc), cp(self.aacc), cp(self.velh), cp(self.wh))
+
+    @classmethod
+    def from_csv(cls, path: str, prefix: str) -> "BallGroup":
+        """Continue a run: the last row of <path><prefix>simData.csv and the constants file
+        (loadSim, ball_group.hpp:1236-1247).  The constants file is <prefix>constants.csv,
+        or the Generator's <prefix>Constants.csv where only that exists."""
+        pos, w, vel = csv_parse_sim_row(csv_last_line(path + prefix + "simData.csv"))
+        consts = path + prefix + "constants.csv"
+        if not os.path.exists(consts) and os.path.exists(path + prefix + "Constants.csv"):
+            consts = path + prefix + "Constants.csv"
+        with open(consts, "rb") as f:
+            R, m, moi = csv_parse_constants(f.read().decode(), len(pos))
+        return cls(pos, vel, w, R, m, moi)
+
+    def to_origin(self) -> "BallGroup":
+        """Moves the centre of mass to the origin, in place (to_origin, sl_to_origin)."""
+        self.pos = _f64(self.pos, (self.num_particles, 3)).copy()
+        rc = lib().sl_to_origin(self.num_particles, _dp(self.pos), _dp(_f64(self.m, (self.num_particles,))))
+        if rc:
+            raise SpacelabError(rc, "sl_to_origin")
+        return self
+
+
+V_CUSTOM = 0.36301555459799423     # dust_const.hpp:34
+
+
+def two_cluster(target: BallGroup, projectile: BallGroup, v_impact: float = V_CUSTOM,
+                impact_angle: float = 0.0) -> tuple[BallGroup, tuple[float, float]]:
+    """The reference's two-cluster set-up of two loaded groups (sl_two_cluster): the
+    merged group (target first) and the (projectile, target) speeds along x."""
+    nt, npj = target.num_particles, projectile.num_particles
+    n = nt + npj
+    out = [np.zeros((n, 3)) for _ in range(3)] + [np.zeros(n) for _ in range(3)]
+    v2 = np.zeros(2)
+
+    def arrays(g, k):
+        return [_f64(g.pos, (k, 3)), _f64(g.vel, (k, 3)), _f64(g.w, (k, 3)), _f64(g.R, (k,)),
+                _f64(g.m, (k,)), _f64(g.moi, (k,))]
+    ta, pa = arrays(target, nt), arrays(projectile, npj)
+    rc = lib().sl_two_cluster(nt, *[_dp(a) for a in ta], npj, *[_dp(a) for a in pa], float(v_impact),
+                              float(impact_angle), *[_dp(a) for a in out], _dp(v2))
+    if rc:
+        raise SpacelabError(rc, "sl_two_cluster")
+    return BallGroup(*out), (float(v2[0]), float(v2[1]))
 
 
 def generate_bpca(n: int, seed: int, r_med: float = 1e-5, sigma: float = 0.2,
@@ -753,6 +801,29 @@ def csv_parse_constants(text: str, n: int):
     if rc:
         raise SpacelabError(rc, "sl_csv_parse_constants")
     return R, m, moi
+
+
+def csv_energy_columns(header: str) -> int:
+    """6 or 9: the columns of an energy.csv file with this header line."""
+    k = lib().sl_csv_energy_columns(header.encode())
+    if k < 0:
+        raise SpacelabError(k, "sl_csv_energy_columns")
+    return k
+
+
+def csv_read_energy(path: str) -> np.ndarray:
+    """Rows of an energy.csv file in either layout (Time,PE,KE,E,p,L[,Bound,Unbound,m]),
+    one row per line after the header."""
+    with open(path, "rb") as f:
+        lines = f.read().decode().split("\n")
+    k = csv_energy_columns(lines[0])
+    rows = [ln for ln in lines[1:] if ln.strip()]
+    out = np.zeros((len(rows), k))
+    for i, ln in enumerate(rows):
+        rc = lib().sl_csv_parse_energy_row(ln.encode(), k, _dp(out[i]))
+        if rc:
+            raise SpacelabError(rc, f"sl_csv_parse_energy_row({path}:{i + 2})")
+    return out
 
 
 def collider_program() -> str:

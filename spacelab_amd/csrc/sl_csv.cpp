@@ -171,10 +171,40 @@ int sl_csv_parse_constants(const char* text, int n, double* R, double* m, double
     if (!text || n < 1 || !R || !m || !moi) return SL_E_ARG;
     const char* p = text;
     for (int b = 0; b < n; b++) {
-        if (!field(p, R + b) || !field(p, m + b) || !field(p, moi + b)) return SL_E_ARG;
-        while (*p == '\r' || *p == '\n') p++;
+        // one line per ball; columns after the third are ignored, as loadConsts reads
+        // three fields of each line
+        const char* eol = p;
+        while (*eol && *eol != '\n') eol++;
+        const std::string line(p, eol);
+        const char* q = line.c_str();
+        if (!field(q, R + b) || !field(q, m + b) || !field(q, moi + b)) return SL_E_ARG;
+        p = *eol ? eol + 1 : eol;
     }
     return SL_OK;
+}
+
+int sl_csv_energy_columns(const char* header)
+{
+    if (!header) return SL_E_ARG;
+    size_t len = 0;
+    while (header[len] && header[len] != '\n' && header[len] != '\r') len++;
+    const std::string h(header, len);
+    if (h == SL_CSV_ENERGY_HEADER) return 6;
+    if (h == SL_CSV_ENERGY_HEADER_BOUND) return 9;
+    return SL_E_ARG;
+}
+
+int sl_csv_parse_energy_row(const char* row, int ncols, double* out)
+{
+    if (!row || !out || (ncols != 6 && ncols != 9)) return SL_E_ARG;
+    const char* p = row;
+    for (int k = 0; k < ncols; k++) {
+        if (!field(p, out + k)) return SL_E_ARG;
+        // the row ends after its last column
+        if (k + 1 < ncols && (p == row || p[-1] != ',')) return SL_E_ARG;
+    }
+    while (*p == '\r' || *p == '\n') p++;
+    return *p ? SL_E_ARG : SL_OK;
 }
 
 long long sl_csv_last_line(const char* path, char* buf, long long cap)
@@ -208,6 +238,9 @@ long long sl_csv_last_line(const char* path, char* buf, long long cap)
                 break;
             }
     }
+    // getLastLine stops its backward scan at the second byte: when the only '\n' left is
+    // the first byte, it reads the first line, which is empty
+    if (found && start == 1) start = size;
     std::string line((size_t)(size - start), '\0');
     std::fseek(f, start, SEEK_SET);
     const size_t got = line.empty() ? 0 : std::fread(&line[0], 1, line.size(), f);

@@ -16,17 +16,16 @@
 //              to slot J of the balls of block I and slot I of the balls of block J
 //              (system-local block numbers), so every (ball, slot) is written by
 //              exactly one tile per step and nothing needs clearing.
-//              After the sweep, wavefront 0 of the tile evaluates the tile's near
-//              pairs with the reference arithmetic, once each, and writes their sums
-//              for both balls to the same slots (near records, read only where the
-//              partner mask is nonzero), so the near-pair latency overlaps other
-//              tiles' far-field work.
 //   k_ball     one workgroup per 64-ball block, eight wavefronts with lane l
-//              working for ball l: sum the ball's slots in a fixed order (far field
-//              Kahan-compensated in the mixed mode, plus the near records); then
-//              wavefront 0 forms acc and aacc, applies the closing kick, the
-//              write-step energies and the next step's drift (writing the other
-//              state set).
+//              working for ball l: each wavefront sums the far-field partials of
+//              every eighth slot (Kahan-compensated in the mixed mode) and counts the
+//              ball's partner-mask bits; the block numbers its near pairs ball by
+//              ball and evaluates them with the reference arithmetic, one per thread,
+//              in rounds of up to 512; then wavefront 0 adds the near results of
+//              each ball in entry order, forms acc and aacc, applies the closing
+//              kick, the write-step energies and the next step's drift (writing the
+//              other state set).  A near pair is evaluated once for each of its two
+//              balls, by the blocks that hold them.
 // Every sum has a fixed order, so results are reproducible run to run.
 #pragma once
 

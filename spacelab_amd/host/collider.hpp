@@ -3,8 +3,9 @@
 // the run's constants, simData and energy files in the reference's layout.
 //
 // Reference counterparts: Ball_group (ball_group.hpp:21-53 arrays, :1236-1247 and
-// :1352-1364 loadSim/sim_continue via getLastLine/parseSimData/loadConsts, :421-516
-// sim_init_write, :519-547 getCOM/to_origin, :1117-1124 getMass), calibrate_dt's first call
+// :1352-1364 loadSim/sim_continue via getLastLine/parseSimData/loadConsts, :1368-1442
+// sim_init_two_cluster, :421-516 sim_init_write, :519-547 getCOM/to_origin, :1117-1124
+// getMass), calibrate_dt's first call
 // (:197-238: updateDTK at a custom speed, then steps and skip from the simulated time
 // and the time resolution) and the driver's sim_one_step / sim_looper
 // (ColliderSingleCore.cpp:78-288, :291-384).
@@ -98,9 +99,12 @@ public:
         Ball_group g(sl_csv_count_balls(line.c_str()));
         int rc = sl_csv_parse_sim_row(line.c_str(), g.num_particles, &g.pos[0].x, &g.w[0].x, &g.vel[0].x);
         if (rc) throw Error(rc, "bad simData row in " + sim);
-        const std::string consts = read_file(path + prefix + "constants.csv");
+        // <prefix>constants.csv, or the Generator's <prefix>Constants.csv where only that exists
+        std::string cname = path + prefix + "constants.csv";
+        if (!file_exists(cname) && file_exists(path + prefix + "Constants.csv")) cname = path + prefix + "Constants.csv";
+        const std::string consts = read_file(cname);
         rc = sl_csv_parse_constants(consts.c_str(), g.num_particles, g.R.data(), g.m.data(), g.moi.data());
-        if (rc) throw Error(rc, "bad constants file " + path + prefix + "constants.csv");
+        if (rc) throw Error(rc, "bad constants file " + cname);
         return g;
     }
 
@@ -139,12 +143,25 @@ public:
 
     void to_origin()
     {
-        const vec3 c = getCOM();
-        for (auto& p : pos) {
-            p.x -= c.x;
-            p.y -= c.y;
-            p.z -= c.z;
-        }
+        const int rc = sl_to_origin(num_particles, &pos[0].x, m.data());
+        if (rc) throw Error(rc, "to_origin: the group has no mass");
+    }
+
+    // The two-cluster set-up (sim_init_two_cluster, ball_group.hpp:1368-1442, over
+    // sl_two_cluster): the projectile offset beside the target at the impact angle,
+    // moving at -v_impact along x, the target at the balancing speed; target first.
+    static Ball_group two_cluster(const Ball_group& target, const Ball_group& projectile, double v_impact,
+                                  double impact_angle, double* v2 = nullptr)
+    {
+        Ball_group g(target.num_particles + projectile.num_particles);
+        const int rc = sl_two_cluster(target.num_particles, &target.pos[0].x, &target.vel[0].x, &target.w[0].x,
+                                      target.R.data(), target.m.data(), target.moi.data(), projectile.num_particles,
+                                      &projectile.pos[0].x, &projectile.vel[0].x, &projectile.w[0].x,
+                                      projectile.R.data(), projectile.m.data(), projectile.moi.data(), v_impact,
+                                      impact_angle, &g.pos[0].x, &g.vel[0].x, &g.w[0].x, g.R.data(), g.m.data(),
+                                      g.moi.data(), v2);
+        if (rc) throw Error(rc, "sl_two_cluster");
+        return g;
     }
 
     void zeroVel()

@@ -1,21 +1,25 @@
 // spacelab_collider: a ColliderSingleCore-shaped driver on the GPU.
 //
 // Builds the Ball_group (continued from a run's CSV files, as
-// Ball_group(path, filename, customVel) does, ball_group.hpp:92-98, or generated),
+// Ball_group(path, filename, customVel) does, ball_group.hpp:92-98; two loaded groups set
+// up for a collision, as Ball_group(path, projectile, target, customVel) does, :100-112
+// and :1368-1442; or generated),
 // calibrates kin, kout, dt, steps and skip as calibrate_dt does (ball_group.hpp:197-238:
 // at --v-custom when it is positive, else by the velocity rule; a continued run defaults
-// to the velocity rule like the reference's continue constructor, a generated group to
-// dust_const.hpp's v_custom), centres it on its centre of mass, runs init_conditions,
+// to the velocity rule like the reference's continue constructor, a generated group and a
+// collision to dust_const.hpp's v_custom, ColliderSingleCore.cpp:32-34), centres it on its centre of mass, runs init_conditions,
 // sim_init_write and sim_looper (ColliderSingleCore.cpp:291-384) and writes
 // <out>constants.csv, <out>simData.csv and <out>energy.csv.  Explicit --dt/--kin/--kout
 // and --steps/--skip override the calibration; --dynamic-time recalibrates at write
 // steps (the reference's dynamicTime).
 //
-//   spacelab_collider (--continue PATH PREFIX | --generate N SEED) [--out PREFIX]
-//       [--v-custom V] [--cor C] [--dt DT --kin KIN --kout KOUT] [--steps N] [--skip K]
+//   spacelab_collider (--continue PATH PREFIX | --two-cluster PATH PROJECTILE TARGET |
+//       --generate N SEED) [--out PREFIX] [--v-impact V] [--impact-angle A] [--v-custom V] [--cor C] [--dt DT --kin KIN --kout KOUT] [--steps N] [--skip K]
 //       [--order ordered|fast] [--precision fp64|mixed] [--preset reference|config]
 //       [--no-center] [--dynamic-time] [--timing] [--quiet]
 //
+// --v-impact (default dust_const.hpp:34 v_custom) and --impact-angle (radians, default
+// dust_const.hpp:36 impactParameter = 0) set the two-cluster collision.
 // --timing prints the wall time of sim_looper and the final flush (write steps, state
 // downloads and CSV output included) on stdout.
 #include "collider.hpp"
@@ -30,8 +34,8 @@ namespace {
 [[noreturn]] void usage(const char* msg)
 {
     std::fprintf(stderr, "spacelab_collider: %s\n"
-                         "usage: spacelab_collider (--continue PATH PREFIX | --generate N SEED) [--out PREFIX]\n"
-                         "  [--v-custom V] [--cor C] [--dt DT --kin KIN --kout KOUT] [--steps N] [--skip K]\n"
+                         "usage: spacelab_collider (--continue PATH PREFIX | --two-cluster PATH PROJECTILE TARGET |\n"
+                         "  --generate N SEED) [--out PREFIX] [--v-impact V] [--impact-angle A] [--v-custom V] [--cor C] [--dt DT --kin KIN --kout KOUT] [--steps N] [--skip K]\n"
                          "  [--order ordered|fast] [--precision fp64|mixed] [--preset reference|config]\n"
                          "  [--no-center] [--dynamic-time] [--timing] [--quiet]\n",
                  msg);
@@ -43,14 +47,16 @@ namespace {
 int main(int argc, char** argv)
 {
     using namespace spacelab;
-    std::string path, prefix, out = "Unnamed_";   // output_prefix default, dust_const.hpp:67
+    std::string path, prefix, projectile, out = "Unnamed_";   // output_prefix default, dust_const.hpp:67
     int gen_n = 0;
     unsigned long long gen_seed = 0;
+    constexpr double kVCustom = 0.36301555459799423;   // dust_const.hpp:34
     double v_custom = -1;                         // unset: chosen by the mode below
+    double v_impact = kVCustom, impact_angle = 0;   // dust_const.hpp:34,36
     double cor = 0.4;                             // dust_const.hpp:27
     double dt = 0, kin = 0, kout = 0;
     int steps = -1, skip = -1, order = SL_ORDER_ORDERED, precision = SL_PREC_FP64, preset = 0;
-    bool center = true, quiet = false, cont = false, dynamic = false, timing = false;
+    bool center = true, quiet = false, cont = false, two = false, dynamic = false, timing = false;
     for (int i = 1; i < argc; i++) {
         const std::string a = argv[i];
         auto next = [&]() -> const char* {
@@ -61,11 +67,18 @@ int main(int argc, char** argv)
             path = next();
             prefix = next();
             cont = true;
+        } else if (a == "--two-cluster") {
+            path = next();
+            projectile = next();
+            prefix = next();
+            two = true;
         } else if (a == "--generate") {
             gen_n = std::atoi(next());
             gen_seed = std::strtoull(next(), nullptr, 0);
         } else if (a == "--out") out = next();
         else if (a == "--v-custom") v_custom = std::atof(next());
+        else if (a == "--v-impact") v_impact = std::atof(next());
+        else if (a == "--impact-angle") impact_angle = std::atof(next());
         else if (a == "--cor") cor = std::atof(next());
         else if (a == "--dt") dt = std::atof(next());
         else if (a == "--kin") kin = std::atof(next());
@@ -90,14 +103,17 @@ int main(int argc, char** argv)
         else if (a == "--quiet") quiet = true;
         else usage(("unknown argument " + a).c_str());
     }
-    if (cont == (gen_n > 0)) usage("give exactly one of --continue and --generate");
+    if ((int)cont + (int)two + (int)(gen_n > 0) != 1) usage("give exactly one of --continue, --two-cluster and --generate");
     try {
-        Ball_group O = cont ? Ball_group::from_csv(path, prefix) : Ball_group::generate(gen_n, gen_seed);
+        Ball_group O = cont  ? Ball_group::from_csv(path, prefix)
+                       : two ? Ball_group::two_cluster(Ball_group::from_csv(path, prefix),
+                                                       Ball_group::from_csv(path, projectile), v_impact, impact_angle)
+                             : Ball_group::generate(gen_n, gen_seed);
         sl_params p;
         sl_params_preset(&p, preset);
         // the continue constructor calibrates with customVel 0 (the velocity rule), the
-        // generator at v_custom; both before centring (ball_group.hpp:67-98)
-        if (v_custom < 0) v_custom = cont ? 0.0 : 0.36301555459799423;   // dust_const.hpp:34
+        // generator and the collision at v_custom; all before centring (ball_group.hpp:67-112)
+        if (v_custom < 0) v_custom = cont ? 0.0 : kVCustom;
         RunControl rc(O, p, cor);
         if (!(dt > 0 && kin > 0 && kout > 0 && steps >= 0 && skip > 0)) rc.calibrate(O, 0, v_custom);
         if (center) O.to_origin();

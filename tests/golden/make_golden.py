@@ -11,7 +11,9 @@ read.
 
 It also keeps the reference's own CSV files of a short run (``csv_cluster16/``:
 the input files a continued run starts from and the files the reference's
-sim_init_write and sim_looper write), for the CSV drop-in tests.
+sim_init_write and sim_looper write), for the CSV drop-in tests; the reference's
+calibrate_dt on four groups (``calib/``); and the reference's two-cluster
+constructor on three pairs of run files (``two_cluster/``).
 
 Cases follow the golden recipe for this path: two spheres head-on (loading then
 unloading), two spheres oblique with spin (sliding and rolling friction), three
@@ -196,11 +198,48 @@ def make_calib_cases():
         print(f"calib/{name}: kin {rec[0, 0]:.6g} steps {rec[0, 3]:.0f} skip {rec[0, 4]:.0f}")
 
 
+TWO_CLUSTER = "two_cluster"
+V_CUSTOM = 0.36301555459799423  # dust_const.hpp:34
+# (case, projectile prefix, customVel of the constructor): the collision constructor at
+# v_custom as ColliderSingleCore.cpp:32 calls it, the velocity rule, and a one-sphere dart
+TWO_CLUSTER_CASES = (("agg_vcustom", "proj_", V_CUSTOM), ("agg_velrule", "proj_", 0.0),
+                     ("dart_vcustom", "dart_", V_CUSTOM))
+
+
+def make_two_cluster_cases():
+    """The reference's own two-cluster constructor (ref_harness --two-cluster) on run
+    files written in the reference's layout: a 60-sphere target, a 12-sphere projectile
+    and a one-sphere dart, each with its own motion and spins.  Stored in
+    tests/golden/two_cluster/: the run files and, per case, the merged group and its run
+    control."""
+    import spacelab_amd as sl
+    d = os.path.join(OUT, TWO_CLUSTER)
+    os.makedirs(d, exist_ok=True)
+    rng = np.random.default_rng(2024)
+    groups = {"targ_": sl.generate_bpca(60, seed=0x7A26, v_sigma=0.02),
+              "proj_": sl.generate_bpca(12, seed=0x9807, v_sigma=0.02),
+              "dart_": sl.generate_bpca(1, seed=0xDA27, v_sigma=0.0)}
+    for prefix, g in groups.items():
+        g.w = rng.standard_normal(g.pos.shape) * 0.5
+        g.pos = g.pos + rng.standard_normal(3) * 1e-4      # off the origin, as a run leaves it
+        with open(os.path.join(d, prefix + "simData.csv"), "w") as f:
+            f.write(sl.csv_sim_header(g.num_particles) + "\n" + sl.csv_sim_row(g.pos, g.w, g.vel))
+        with open(os.path.join(d, prefix + "constants.csv"), "w") as f:
+            f.write(sl.csv_constants(g.R, g.m, g.moi))
+    for name, proj, custom in TWO_CLUSTER_CASES:
+        r = oracle.run_reference_two_cluster(d, proj, "targ_", custom)
+        np.savez(os.path.join(d, name + ".npz"), projectile=np.array(proj), target=np.array("targ_"),
+                 custom_speed=np.array(custom), run=r["run"], fields=np.array(oracle.TWO_CLUSTER_FIELDS),
+                 pos=r["pos"], vel=r["vel"], w=r["w"], R=r["R"], m=r["m"], moi=r["moi"])
+        print(f"{TWO_CLUSTER}/{name}: n={len(r['R'])} kin {r['run'][0]:.6g} dt {r['run'][2]:.6g}")
+
+
 def main():
     if not oracle.ref_available():
         oracle.build()
     if not oracle.ref_available():
         raise SystemExit("reference harness not built (reference sources absent)")
+    make_two_cluster_cases()
     make_calib_cases()
     make_csv_case()
     for name, (fn, runs) in CASES.items():

@@ -86,6 +86,12 @@ def test_last_line_edges(tmp_path):
     assert sl.csv_last_line(str(f)) == "only"
     f.write_bytes(b"x\n")
     assert sl.csv_last_line(str(f)) == ""
+    # getLastLine's backward scan stops at the second byte: a file whose only newline is
+    # its first byte reads as its (empty) first line
+    f.write_bytes(b"\nrow")
+    assert sl.csv_last_line(str(f)) == ""
+    f.write_bytes(b"\n\nrow")
+    assert sl.csv_last_line(str(f)) == "row"
     big = b"h\n" + b"1," * 10000 + b"9"                     # longer than one read block
     f.write_bytes(big)
     assert sl.csv_last_line(str(f)) == big.decode().split("\n")[-1]

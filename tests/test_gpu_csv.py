@@ -126,3 +126,30 @@ def test_driver_dynamic_time(tmp_path):
     assert len(en) == 3                      # header, initial row, the row of step 10
     sim = open(tmp_path / "dyn_simData.csv").read().split("\n")
     assert len(sim) == 3
+
+
+def test_driver_two_cluster(tmp_path):
+    """--two-cluster loads a projectile and a target run and sets up their collision as
+    the reference's collision constructor does (ColliderSingleCore.cpp:32,
+    ball_group.hpp:100-112).  The merged, calibrated and centred group is pinned bitwise
+    to the reference's own constructor (tests/golden/two_cluster/agg_vcustom.npz), so the
+    driver's constants file and first simData row are that group's bytes, and its dt and
+    k are the reference's."""
+    d = os.path.join(GOLDEN, "two_cluster") + os.sep
+    z = np.load(d + "agg_vcustom.npz")
+    cmd = [sl.collider_program(), "--two-cluster", d, "proj_", "targ_", "--out", str(tmp_path) + "/tc_",
+           "--steps", "21", "--skip", "10", "--order", "ordered"]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    run = dict(zip([str(f) for f in z["fields"]], z["run"]))
+    assert f"dt: {run['dt']:.6g}\n" in r.stderr and f"k: {run['kin']:.6g}\n" in r.stderr
+    assert open(tmp_path / "tc_constants.csv").read() == sl.csv_constants(z["R"], z["m"], z["moi"])
+    sim = open(tmp_path / "tc_simData.csv").read().split("\n")
+    assert sim[0] == sl.csv_sim_header(72) and len(sim) == 4
+    assert sim[1] == sl.csv_sim_row(z["pos"], z["w"], z["vel"])
+    for row in sim[2:]:
+        pos, w, vel = sl.csv_parse_sim_row(row)
+        assert np.all(np.isfinite(pos)) and np.all(np.isfinite(vel))
+    e = sl.csv_read_energy(str(tmp_path / "tc_energy.csv"))
+    assert e.shape == (3, 6) and np.all(np.isfinite(e))

@@ -16,7 +16,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import bits_equal, golden_names, load_golden
+from conftest import GOLDEN, bits_equal, golden_names, load_golden
 
 import spacelab_amd as sl
 
@@ -222,3 +222,62 @@ def test_many_partners_per_sphere(oracle_mod):
     s, _, _ = oracle_mod.run(st, P, 21, 0, mode="gather", nthreads=NTHREADS)
     assert relerr(out.pos, s.pos) <= 1e-12
     assert relerr(out.vel, s.vel) <= 1e-10
+
+
+def test_fast_40000_vs_oracle(oracle_mod):
+    """A 40,000-sphere aggregate: 625 ball blocks, so a wavefront of the per-ball kernel
+    visits more than 64 partial slots and numbers its near entries by rescanning every
+    slot's partner masks rather than from its 64-bit list of slots with partners.  The
+    step-0 pass and one step against the oracle.  With this many contacts some are
+    ill-conditioned: the friction direction of a contact with nearly cancelling relative
+    spin turns rounding into ~1e-11 changes of its torques, which reach spins and, through
+    the kick, velocities.  So each array's bar is the one-step bar above or ten times the
+    oracle's own change when only the order of the spheres (and so of every sum) is
+    permuted, whichever is larger (measured: 1.9e-14 acc, 1.4e-14 vel, 6.5e-12 aacc and w,
+    the GPU's differences being of the same size)."""
+    n = 40000
+    assert (n + 63) // 64 > 64 * 8
+    g = sl.generate_bpca(n, seed=0x5EED0040, v_sigma=1e-3)
+    p = sl.calibrate(sl.preset("config"), g.R, g.m, velocity=V_CUSTOM, cor=0.64).replace(
+        order=sl.ORDER_FAST)
+    P = _oracle_params(oracle_mod, p)
+    out1, e0, _ = _run(g, p, 1)
+    s1, oe0, _ = oracle_mod.run(oracle_mod.State(g.pos, g.vel, g.w, g.R, g.m, g.moi), P, 2, 0,
+                                mode="gather", nthreads=NTHREADS)
+    perm = np.random.default_rng(0).permutation(n)
+    sp, _, _ = oracle_mod.run(oracle_mod.State(g.pos[perm], g.vel[perm], g.w[perm], g.R[perm], g.m[perm],
+                                               g.moi[perm]), P, 2, 0, mode="gather", nthreads=NTHREADS)
+    inv = np.argsort(perm)
+    assert e0.contacts > 0
+    for k, bar in (("acc", 1e-13), ("vel", 1e-14), ("aacc", 1e-13), ("w", 1e-13), ("pos", 1e-15)):
+        cond = relerr(getattr(sp, k)[inv], getattr(s1, k))
+        assert relerr(getattr(out1, k), getattr(s1, k)) <= max(bar, 10 * cond), (k, cond)
+    assert abs(e0.PE - oe0[0]) <= 1e-12 * abs(oe0[0])
+
+
+def test_generator_aggregate_fast_and_mixed(oracle_mod):
+    """The 1000-sphere aggregate the reference's Generator wrote (the first row of its
+    simData file, tests/golden/generator/): three radii of 1481-4443 cm, masses up to
+    6e11 g, every pair apart by at least 1e-4 of its radii -- a far-field input at
+    astronomical scale.  Fast path: the step-0 pass and one step against the oracle to the
+    one-step bars.  Mixed precision (FP32 far field in scaled units): energy within 1e-6
+    of FP64 over 100 steps."""
+    d = os.path.join(GOLDEN, "generator") + os.sep
+    prefix = "1000-R29019.1-k1e+18-cor0.64-mu0.3-rho1.70-dt0.04_"
+    consts = sl.BallGroup.from_csv(d, prefix)
+    with open(d + prefix + "simData.csv", "rb") as f:
+        pos, w, vel = sl.csv_parse_sim_row(f.read().decode().split("\n")[1])
+    g = sl.BallGroup(pos, vel, w, consts.R, consts.m, consts.moi)
+    p = sl.calibrate(sl.preset("config"), g.R, g.m, velocity=V_CUSTOM, cor=0.64).replace(
+        order=sl.ORDER_FAST)
+    out1, e0, _ = _run(g, p, 1)
+    s1, oe0, _ = oracle_mod.run(oracle_mod.State(g.pos, g.vel, g.w, g.R, g.m, g.moi),
+                                _oracle_params(oracle_mod, p), 2, 0, mode="gather", nthreads=NTHREADS)
+    assert relerr(out1.acc, s1.acc) <= 1e-13
+    assert relerr(out1.vel, s1.vel) <= 1e-14
+    assert relerr(out1.pos, s1.pos) <= 1e-15
+    assert abs(e0.PE - oe0[0]) <= 1e-12 * abs(oe0[0])
+    _, e0f, ewf = _run(g, p, 100, 10)
+    _, e0m, ewm = _run(g, p.replace(precision=sl.PREC_MIXED), 100, 10)
+    for a, b in zip([e0m] + ewm, [e0f] + ewf):
+        assert abs(a.E - b.E) <= 1e-6 * (abs(b.PE) + b.KE)
