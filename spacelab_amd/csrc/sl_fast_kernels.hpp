@@ -444,6 +444,19 @@ __global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
     }
 }
 
+__device__ __forceinline__ D3 load3(const State& c, int f, int b)
+{
+    return d3(c.f[f][b], c.f[f + 1][b], c.f[f + 2][b]);
+}
+
+// Result of one (ball, partner) near evaluation, for the ball: force, torque, and the
+// pair's energy and touching flag when the ball is A (counted once per pair).
+struct NearRes {
+    double f[6];
+    double pe;
+    int contact, is_a;
+};
+
 // Velocities, spins and previous positions of a pair's two balls, read only when
 // the pair touches.
 struct NearStateG {
@@ -457,19 +470,6 @@ struct NearStateG {
     {
         return norm(d3(s.f[QX][B], s.f[QY][B], s.f[QZ][B]) - d3(s.f[QX][A], s.f[QY][A], s.f[QZ][A]));
     }
-};
-
-__device__ __forceinline__ D3 load3(const State& c, int f, int b)
-{
-    return d3(c.f[f][b], c.f[f + 1][b], c.f[f + 2][b]);
-}
-
-// Result of one (ball, partner) near evaluation, for the ball: force, torque, and the
-// pair's energy and touching flag when the ball is A (counted once per pair).
-struct NearRes {
-    double f[6];
-    double pe;
-    int contact, is_a;
 };
 
 // Evaluate the near pair of ball b and partner p with the reference arithmetic in the
@@ -521,12 +521,14 @@ constexpr int KB_W = 8;              // wavefronts per k_ball block (one 64-ball
 constexpr int KB_T = KB_W * TB;
 constexpr int BPB = TB;              // balls per k_ball block
 constexpr int SLOTS_IN_FLIGHT = 8;   // slots per lane loaded together
+constexpr int KB_MC = 4;             // partner masks kept per (wavefront, ball)
 
 // One 64-ball block per workgroup; lane l of every wavefront works for ball l.
 //
 // 1. Wavefront w sums the far-field partials of the slots K = w, w + 8, ... in order
-//    (reads of 64 consecutive doubles; Kahan-compensated in the mixed mode) and counts
-//    the near-partner bits of those slots.
+//    (reads of 64 consecutive doubles; Kahan-compensated in the mixed mode), counts
+//    the near-partner bits of those slots and keeps the first KB_MC nonzero partner
+//    masks of each ball in LDS (more than that: they are read again).
 // 2. The block's near entries are numbered ball by ball (then by wavefront, slot and
 //    bit), and all 512 threads evaluate them, KB_T at a time, with the reference
 //    arithmetic and orientation.  Lane l of wavefront 0 adds its ball's results in
@@ -543,6 +545,8 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
     __shared__ int s_total;
     __shared__ int ent[KB_T];                // entries of the current round
     __shared__ NearRes res[KB_T];
+    __shared__ unsigned long long s_mk[KB_W][KB_MC][TB];   // nonzero partner masks ...
+    __shared__ int s_mks[KB_W][KB_MC][TB];                  // ... and their slots
     const int w = threadIdx.x / TB, l = threadIdx.x % TB;
     const int i = blockIdx.x * TB + l;
     const int sys = a.blk_sys[blockIdx.x];
@@ -574,8 +578,7 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
 
     double fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
     int cn = 0;                    // near entries of this (wavefront, ball)
-    unsigned long long nzs = 0;    // bit o: slot w + 8 o has near partners
-    bool many = false;             // more than 64 slots per wavefront: rescan them all
+    int nm = 0;                    // slots of this (wavefront, ball) with near partners
     for (int K0 = w; K0 < nblk; K0 += SLOTS_IN_FLIGHT * KB_W) {
         double q[SLOTS_IN_FLIGHT][3];
         unsigned long long mk[SLOTS_IN_FLIGHT];
@@ -594,11 +597,11 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
             if (K0 + u * KB_W >= nblk) break;
             cn += __popcll(mk[u]);
             if (mk[u]) {
-                const int o = (K0 - w) / KB_W + u;
-                if (o < 64)
-                    nzs |= 1ull << o;
-                else
-                    many = true;
+                if (nm < KB_MC) {
+                    s_mk[w][nm][l] = mk[u];
+                    s_mks[w][nm][l] = K0 + u * KB_W;
+                }
+                nm++;
             }
             if (COMP) {
                 double y, t;
@@ -646,8 +649,7 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
         // write this round's entries of (wavefront w, ball l)
         if (cn && my0 < r1 && my0 + cn > r0) {
             int k = my0;
-            auto put = [&](int K) {
-                unsigned long long m = a.nbm[(size_t)K * np + ic];
+            auto put = [&](int K, unsigned long long m) {
                 while (m && k < r1) {
                     const int bit = __ffsll((long long)m) - 1;
                     m &= m - 1;
@@ -655,11 +657,10 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
                     k++;
                 }
             };
-            if (!many) {
-                // only the slots that had partners (their masks are still in cache)
-                for (unsigned long long z = nzs; z && k < r1; z &= z - 1) put(w + (__ffsll((long long)z) - 1) * KB_W);
+            if (nm <= KB_MC) {
+                for (int q = 0; q < nm && k < r1; q++) put(s_mks[w][q][l], s_mk[w][q][l]);
             } else {
-                for (int K = w; K < nblk && k < r1; K += KB_W) put(K);
+                for (int K = w; K < nblk && k < r1; K += KB_W) put(K, a.nbm[(size_t)K * np + ic]);
             }
         }
         __syncthreads();

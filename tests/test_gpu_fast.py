@@ -228,13 +228,16 @@ def test_fast_40000_vs_oracle(oracle_mod):
     """A 40,000-sphere aggregate: 625 ball blocks, so a wavefront of the per-ball kernel
     visits more than 64 partial slots and numbers its near entries by rescanning every
     slot's partner masks rather than from its 64-bit list of slots with partners.  The
-    step-0 pass and one step against the oracle.  With this many contacts some are
-    ill-conditioned: the friction direction of a contact with nearly cancelling relative
-    spin turns rounding into ~1e-11 changes of its torques, which reach spins and, through
-    the kick, velocities.  So each array's bar is the one-step bar above or ten times the
-    oracle's own change when only the order of the spheres (and so of every sum) is
-    permuted, whichever is larger (measured: 1.9e-14 acc, 1.4e-14 vel, 6.5e-12 aacc and w,
-    the GPU's differences being of the same size)."""
+    step-0 pass and one step against the oracle.  With this many contacts some touch by
+    only a few 1e-6 of their radii, where the overlap S - d cancels: a one-ulp change of a
+    drifted position (from the rounding of the forces that drifted it) changes the
+    elastic force, and the friction torque proportional to it, by some 1e-11.  So each
+    array's bar is the one-step bar above or ten times the oracle's own change when only
+    the numbering of the spheres (and so the order of every sum) is permuted, whichever
+    is larger.  Spins are zero in this step's pair pass, so renumbering, which swaps the
+    roles A and B of some pairs, changes nothing but rounding.  Measured: 1.9e-14 acc,
+    1.4e-14 vel, 6.5e-12 aacc and w; the GPU's differences are the same quantized
+    changes, of the same size."""
     n = 40000
     assert (n + 63) // 64 > 64 * 8
     g = sl.generate_bpca(n, seed=0x5EED0040, v_sigma=1e-3)
@@ -281,3 +284,39 @@ def test_generator_aggregate_fast_and_mixed(oracle_mod):
     _, e0m, ewm = _run(g, p.replace(precision=sl.PREC_MIXED), 100, 10)
     for a, b in zip([e0m] + ewm, [e0f] + ewf):
         assert abs(a.E - b.E) <= 1e-6 * (abs(b.PE) + b.KE)
+
+
+def test_sphere_with_4000_partners(oracle_mod):
+    """A sphere pressed against by 4000 small ones spread over its whole surface (a
+    Fibonacci lattice, neighbours 2.4e-8 cm apart, inside the h_min band).  In the
+    engine's space-filling-curve order its partners lie in all 63 ball blocks, so each
+    wavefront of the per-ball kernel meets them in more slots than it keeps partner
+    masks for and numbers them by reading the masks again, over several rounds of 512
+    evaluations in the big sphere's block.  The step-0 pass and one step against the
+    oracle, to the one-step bars; spins are nonzero, so the friction terms that are not
+    symmetric in A and B are exercised with the caller's orientation.  After one step the
+    velocities are the kicks of the compressed contacts (about 0.7 cm/s against 1e-3 cm/s
+    at the start), so their bar is the forces' (1e-13) rather than 1e-14."""
+    k = 4000
+    rs, Rb = 1e-5, 4e-4
+    t = np.arange(k) + 0.5
+    phi, th = np.arccos(1 - 2 * t / k), np.pi * (1 + 5 ** 0.5) * t
+    d = np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], 1)
+    pos = np.vstack([[0.0, 0.0, 0.0], d * (Rb + rs) * (1 - 1e-3)])
+    R = np.array([Rb] + [rs] * k)
+    m = 2.25 * 4 / 3 * np.pi * R ** 3
+    rng = np.random.default_rng(11)
+    vel = rng.standard_normal(pos.shape) * 1e-3
+    w = rng.standard_normal(pos.shape) * 1e-2
+    g = sl.BallGroup(pos, vel, w, R, m, 0.4 * m * R * R)
+    p = sl.calibrate(sl.preset("config"), R, m, velocity=V_CUSTOM, cor=0.64).replace(order=sl.ORDER_FAST)
+    out1, e0, _ = _run(g, p, 1)
+    s1, oe0, _ = oracle_mod.run(oracle_mod.State(g.pos, g.vel, g.w, g.R, g.m, g.moi),
+                                _oracle_params(oracle_mod, p), 2, 0, mode="gather", nthreads=NTHREADS)
+    assert e0.contacts == k
+    assert relerr(out1.acc, s1.acc) <= 1e-13
+    assert relerr(out1.aacc, s1.aacc) <= 1e-13
+    assert relerr(out1.vel, s1.vel) <= 1e-13
+    assert relerr(out1.w, s1.w) <= 1e-13
+    assert relerr(out1.pos, s1.pos) <= 1e-15
+    assert abs(e0.PE - oe0[0]) <= 1e-12 * abs(oe0[0])
