@@ -174,7 +174,7 @@ def energy_scale(e):
     return abs(e.PE) + e.KE
 
 
-def ensemble(n_coll=512, steps=200, warmup=20, horizon=50000, seed=0x5EED0003):
+def ensemble(n_coll=512, steps=200, warmup=100, horizon=50000, seed=0x5EED0003):
     """Batched collision ensemble at full size: target N log-uniform 200-4000,
     projectile N log-uniform 1-500, speed uniform 10-1000 cm/s, b uniform in
     [0, 1] (R_t + R_p), springs and dt calibrated per collision at its impact speed
@@ -198,17 +198,20 @@ def ensemble(n_coll=512, steps=200, warmup=20, horizon=50000, seed=0x5EED0003):
         with sl.Batch(n_coll, int(sizes.sum()), p) as b:
             b.upload(groups, kin=kin, kout=kout, dt=dt)
             e0 = b.init_conditions()
-            b.step(warmup)
+            ew_warm = b.step(warmup, warmup)    # energies at the end of the (untimed) warm-up
             b.download()
             t0 = time.perf_counter()
             ew = b.step(steps, steps)           # energies of every collision at the last step
             b.download()
-            return e0, ew[-1], time.perf_counter() - t0, b.get_option("tile_split")
+            return e0, ew_warm[-1], ew[-1], time.perf_counter() - t0, b.get_option("tile_split")
 
-    e0, e64, sec, split = run(sl.PREC_FP64)
-    _, emx, sec_mx, _ = run(sl.PREC_MIXED)
+    e0, w64, e64, sec, split = run(sl.PREC_FP64)
+    _, wmx, emx, sec_mx, _ = run(sl.PREC_MIXED)
+
     # per collision: |E_mixed - E_fp64| / (|PE| + KE) after the same steps from the same state
-    rel = np.array([abs(a.E - b_.E) / (abs(b_.PE) + b_.KE) for a, b_ in zip(emx, e64)])
+    def rel_of(xs, ys):
+        r = np.array([abs(a.E - b_.E) / (abs(b_.PE) + b_.KE) for a, b_ in zip(xs, ys)])
+        return {"max": float(r.max()), "median": float(np.median(r)), "over_1e-6": int((r > 1e-6).sum())}
     pairs = int((sizes * (sizes - 1) // 2).sum())
     step_s = sec / steps
     return {"collisions": n_coll, "balls": int(sizes.sum()), "balls_min_max": [int(sizes.min()), int(sizes.max())],
@@ -216,8 +219,12 @@ def ensemble(n_coll=512, steps=200, warmup=20, horizon=50000, seed=0x5EED0003):
             "pairs_per_s": pairs / step_s, "horizon_steps": horizon,
             "collisions_per_s": n_coll / (horizon * step_s), "tile_split": split,
             "mixed_ms_per_step": 1e3 * sec_mx / steps,
-            "mixed_vs_fp64_energy_rel": {"max": float(rel.max()), "median": float(np.median(rel)),
-                                         "after_steps": (warmup + steps) // steps * steps},
+            "mixed_vs_fp64_energy_rel": {
+                f"after_{warmup}_steps": rel_of(wmx, w64), f"after_{warmup + steps}_steps": rel_of(emx, e64),
+                "note": "differences grow smoothly at rounding level until a contact touches or its spring "
+                        "switches between loading and unloading one step earlier in one run than in the other; "
+                        "the energy dissipated then differs (any two non-bitwise runs do this, FP64 against "
+                        "FP64 included)"},
             "setup_s": t_gen, "nonfinite": int(sum(e.nonfinite for e in e0)),
             "note": "collisions/s = collisions / (horizon * measured step time); the timed steps "
                     "are a bounded sample of the 50k-step run, inputs resident in HBM"}
@@ -433,7 +440,10 @@ def main():
     r = res[primary]
     fast = "tile_kernel_tflops" in pm
     achieved = pm["tile_kernel_tflops"] if fast else pm["pair_kernel_tflops"]
-    roof = {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+    # compute-bound ("mfma" is the compute roof of the two): the kernel runs on the FP64
+    # vector ALUs, whose MI355X peak equals the FP64 matrix peak, 78.6 TFLOP/s
+    roof = {"bound": "mfma", "pipe": "fp64 vector ALU (no MFMA instructions; same 78.6 TFLOP/s peak)",
+            "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS if achieved else None, "traffic": None,
             "kernel": "k_tiles (far-field tile sweep)" if fast else "k_pairs_ordered",
             "kernel_ms": pm["tile_kernel_ms"] if fast else pm["pair_kernel_ms"],

@@ -11,7 +11,11 @@
 //              of the j it meets next; no LDS atomics or cross-lane reductions.
 //              A pair closer than S + h_min (touching, or in the band where the
 //              reference clamps the vdW gap) contributes nothing to the sweep: it is
-//              only recorded, as a bit of the partner masks of both balls.  Each
+//              only recorded, as a bit of the partner masks of both balls.  The near
+//              test's compare is already the wavefront's lane mask: a step with any
+//              near lane stores that mask in LDS (a uniform branch, rarely taken), and
+//              each lane's partner bits are rebuilt from the stored steps after the
+//              sweep, so the sweep itself spends no vector instructions on them.  Each
 //              tile writes its i-side and j-side force sums and partner masks once,
 //              to slot J of the balls of block I and slot I of the balls of block J
 //              (system-local block numbers), so every (ball, slot) is written by
@@ -221,8 +225,11 @@ __global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
     double (*comb)[6][TB] = reinterpret_cast<double (*)[6][TB]>(lds);
     __shared__ double pe_w[NW];
     __shared__ unsigned long long im[TPB][TB], jm[TPB][TB];
+    __shared__ unsigned long long bal[NW][TB];   // per wavefront: near lanes of step s
 
-    const int w = threadIdx.x / TB, l = threadIdx.x % TB;
+    // the wavefront index is uniform: in a scalar register, the tile and sweep indices
+    // derived from it are too
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / TB), l = threadIdx.x % TB;
     const int tl = w / SPLIT, h = w % SPLIT;
     const int t = blockIdx.x * TPB + tl;
     const bool live = t < a.n_tiles;
@@ -252,7 +259,7 @@ __global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
     __syncthreads();
 
     double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0, pe = 0;
-    unsigned long long nmask = 0;       // bit s: near pair met at step s
+    unsigned long long any = 0;         // bit s: some lane met a near pair at step s
     if (live) {
         const JRec64* rec = &sj[tl][l];
         const double Ri4 = 4 * Ri;
@@ -283,7 +290,11 @@ __global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
                     const bool near = sv < __builtin_fma(S, two_hmin, hmin2);
                     use = act && !near;
                     f = use ? f : 0.f;
-                    nmask |= (act && near) ? (1ull << s) : 0ull;
+                    const unsigned long long b = __ballot(act && near);
+                    if (b) {
+                        if (l == 0) bal[w][s] = b;
+                        any |= 1ull << s;
+                    }
                 }
                 const float dxf = (float)dx * L0inv, dyf = (float)dy * L0inv, dzf = (float)dz * L0inv;
                 fax = __builtin_fmaf(f, dxf, fax);
@@ -347,7 +358,11 @@ __global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
                     const bool near = sv < __builtin_fma(S, two_hmin, hmin2);   // (S + h_min)^2 - S^2
                     use = act && !near;
                     f = use ? f : 0.0;
-                    nmask |= (act && near) ? (1ull << s) : 0ull;
+                    const unsigned long long b = __ballot(act && near);
+                    if (b) {
+                        if (l == 0) bal[w][s] = b;
+                        any |= 1ull << s;
+                    }
                 }
                 ax = __builtin_fma(f, dx, ax);
                 ay = __builtin_fma(f, dy, ay);
@@ -385,6 +400,12 @@ __global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
                 by = rotk(by, back);
                 bz = rotk(bz, back);
             }
+        }
+        // this lane's near steps, from the wavefront's recorded steps
+        unsigned long long nmask = 0;
+        for (unsigned long long z = any; z; z &= z - 1) {
+            const int s = __ffsll((long long)z) - 1;
+            if ((bal[w][s] >> l) & 1ull) nmask |= 1ull << s;
         }
         if (nmask) {
             atomicOr(&im[tl][l], rotl64(nmask, l));
