@@ -320,3 +320,32 @@ def test_sphere_with_4000_partners(oracle_mod):
     assert relerr(out1.w, s1.w) <= 1e-13
     assert relerr(out1.pos, s1.pos) <= 1e-15
     assert abs(e0.PE - oe0[0]) <= 1e-12 * abs(oe0[0])
+
+
+def test_benchmark_aggregate_5000_fast_and_mixed(oracle_mod):
+    """The benchmark's own workload (the 5000-sphere aggregate, seed 0x5EED0001, config
+    physics, dt = t_c/50) through the fast path: the step-0 pass and one step against the
+    oracle to the one-step bars, and 100 steps to the 100-step bars; the mixed path's
+    energy within 1e-6 of the fast FP64 path's over the same 100 steps."""
+    n = 5000
+    g = sl.generate_bpca(n, seed=0x5EED0001, v_sigma=1e-3)
+    p = sl.calibrate(sl.preset("config"), g.R, g.m, velocity=V_CUSTOM, cor=0.64, dt_rule="tc50").replace(
+        order=sl.ORDER_FAST)
+    P = _oracle_params(oracle_mod, p)
+    st = oracle_mod.State(g.pos, g.vel, g.w, g.R, g.m, g.moi)
+    out1, e0, _ = _run(g, p, 1)
+    s1, oe0, _ = oracle_mod.run(st, P, 2, 0, mode="gather", nthreads=NTHREADS)
+    assert relerr(out1.acc, s1.acc) <= 1e-13
+    assert relerr(out1.aacc, s1.aacc) <= 1e-13
+    assert relerr(out1.vel, s1.vel) <= 1e-14
+    assert relerr(out1.pos, s1.pos) <= 1e-15
+    assert abs(e0.PE - oe0[0]) <= 1e-12 * abs(oe0[0])
+    out, _, ew = _run(g, p, 100, 50)
+    s, _, oew = oracle_mod.run(st, P, 101, 50, mode="gather", nthreads=NTHREADS)
+    assert relerr(out.pos, s.pos) <= 1e-12
+    assert relerr(out.vel, s.vel) <= 1e-10
+    for e, r in zip(ew, oew):
+        assert abs(e.E - (r[0] + r[1])) <= 1e-12 * (abs(r[0]) + r[1])
+    _, e0m, ewm = _run(g, p.replace(precision=sl.PREC_MIXED), 100, 50)
+    for a, b in zip([e0m] + ewm, [e0] + ew):
+        assert abs(a.E - b.E) <= 1e-6 * (abs(b.PE) + b.KE)

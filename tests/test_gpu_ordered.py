@@ -94,6 +94,29 @@ def test_bpca_1000_bitwise_100_steps(oracle_mod, far_gravity, v_sigma):
     assert e0.contacts >= 0
 
 
+def test_benchmark_aggregate_5000_bitwise_100_steps(oracle_mod):
+    """The benchmark's own workload: the 5000-sphere aggregate (seed 0x5EED0001, config
+    physics with gravity on every pair, dt = t_c/50), 100 steps through the ordered path:
+    every state array bitwise the oracle's, energies at the write steps to 1e-12."""
+    n = 5000
+    g = sl.generate_bpca(n, seed=0x5EED0001, v_sigma=1e-3)
+    p = sl.calibrate(sl.preset("config"), g.R, g.m, velocity=0.36301555459799423, cor=0.64, dt_rule="tc50")
+    with sl.Collider(n, p) as c:
+        c.upload(g)
+        e0 = c.init_conditions()
+        ew = c.step(100, 50)
+        out = c.download()
+    st = oracle_mod.State(g.pos, g.vel, g.w, g.R, g.m, g.moi)
+    s, oe0, oew = oracle_mod.run(st, _oracle_params(oracle_mod, p), 101, 50, mode="gather",
+                                 nthreads=NTHREADS)
+    for k in ("pos", "vel", "velh", "w", "wh", "acc", "aacc"):
+        assert bits_equal(getattr(out, k), getattr(s, k)), k
+    _energy_close(e0, oe0, g.m, g.moi, g.pos, g.vel, g.w, rtol=1e-12)
+    assert len(ew) == 2
+    for e, r in zip(ew, oew):
+        _energy_close(e, r, g.m, g.moi, out.pos, out.vel, out.w, rtol=1e-12)
+
+
 def test_energy_10k_steps(oracle_mod):
     """10k steps of a 200-sphere aggregate: states bitwise, E/p/L within 1e-9."""
     n = 200
