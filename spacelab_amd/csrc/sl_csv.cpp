@@ -1,11 +1,14 @@
 // Text files of a run in the reference's CSV layout (constants, simData, energy):
 // formatting, parsing and the last-line reader a continued run starts from.  Plain
-// C++ host code.  Numbers go through printf's "%.*g", which is what C++ streams use
-// for a double in the default float field.
+// C++ host code.  Numbers are printed as printf's "%.*g" prints them, which is what C++
+// streams use for a double in the default float field: std::to_chars in the general
+// format with a precision is specified to give exactly that text, about three times
+// faster than snprintf.
 #include "spacelab/sl_csv.h"
 
 #include "spacelab/sl_api.h"
 
+#include <charconv>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -17,8 +20,8 @@ namespace {
 void put(std::string& s, double v, int digits)
 {
     char b[64];
-    const int k = std::snprintf(b, sizeof b, "%.*g", digits, v);
-    s.append(b, (size_t)k);
+    const std::to_chars_result r = std::to_chars(b, b + sizeof b, v, std::chars_format::general, digits);
+    s.append(b, (size_t)(r.ptr - b));
 }
 
 long long emit(const std::string& s, char* buf, long long cap)

@@ -368,38 +368,44 @@ private:
         O.ang_mom = vec3{};
     }
 
+    // One formatting pass into a buffer of `guess` bytes (an upper bound for the text;
+    // a second pass only if the text turns out longer).
     template <class F>
-    static std::string text(F f)
+    static std::string text(F f, size_t guess)
     {
-        const long long len = f(nullptr, 0);
+        std::string s(guess + 1, '\0');
+        long long len = f(&s[0], (long long)s.size());
         if (len < 0) throw Error((int)len, "CSV formatting");
-        std::string s((size_t)len + 1, '\0');
-        f(&s[0], len + 1);
+        if ((size_t)len >= s.size()) {
+            s.assign((size_t)len + 1, '\0');
+            f(&s[0], len + 1);
+        }
         s.resize((size_t)len);
         return s;
     }
 
+    // a number takes at most 13 characters at 6 digits ("-1.23457e-308") and 19 at 12
     std::string sim_header(int n)
     {
-        return text([&](char* b, long long c) { return sl_csv_sim_header(n, b, c); });
+        return text([&](char* b, long long c) { return sl_csv_sim_header(n, b, c); }, (size_t)n * 80 + 16);
     }
     std::string sim_row()
     {
         return text([&](char* b, long long c) {
             return sl_csv_sim_row(O.num_particles, &O.pos[0].x, &O.w[0].x, &O.vel[0].x, b, c);
-        });
+        }, (size_t)O.num_particles * (11 * 14) + 16);
     }
     std::string energy_row(int digits)
     {
         return text([&](char* b, long long c) {
             return sl_csv_energy_row(simTimeElapsed, O.PE, O.KE, &O.mom.x, &O.ang_mom.x, digits, b, c);
-        });
+        }, 6 * 20 + 16);
     }
     std::string constants()
     {
         return text([&](char* b, long long c) {
             return sl_csv_constants(O.num_particles, O.R.data(), O.m.data(), O.moi.data(), b, c);
-        });
+        }, (size_t)O.num_particles * (3 * 14) + 16);
     }
 
     static void append(const std::string& path, const std::string& s)
