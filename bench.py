@@ -355,8 +355,8 @@ def main():
                     help="run the plain copy kernel on this many MiB first (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {a.gpus}); this build does not shard "
-              "across GPUs", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the engine does not shard "
+              "one system across GPUs", file=sys.stderr)
         sys.exit(2)
 
     import spacelab_amd as sl

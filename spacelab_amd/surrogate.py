@@ -4,7 +4,7 @@ entropy-gated inference that hands every collision whose predictive entropy is h
 back to DEM.
 
 The reference has no counterpart: it simulates every collision with DEM, one
-sim_one_step per time step (ColliderSingleCore.cpp:78-288).  Here DEM is the judge of
+sim_one_step per time step (ColliderSingleCore.cpp:78-288).  Here DEM is the arbiter of
 the surrogate: its labels train the network and decide every collision the network
 is unsure of.
 
@@ -282,7 +282,7 @@ def evaluate(train: dict, heldout: dict, target: float = 0.95, z: float = 2.0, v
              epochs: int = 40, lr: float = 1e-3, n_models: int = 4, seed: int = 1, log=None) -> dict:
     """Train on ``train`` with a validation share held back to set the entropy gate,
     then run the held-out collisions twice, by DEM alone and by the gated surrogate,
-    with live DEM as the judge of the second.
+    with live DEM deciding the true outcome of the second.
 
     The gate aims above ``target`` by z standard errors of the difference between two
     binomial estimates, the validation share's and the held-out set's, so that the

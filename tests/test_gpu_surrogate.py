@@ -1,7 +1,7 @@
 """The learned collision surrogate end to end on the GPU: the perceptron is trained by
 the library's own kernels on the committed DEM-labelled collisions, its entropy gate
 is set on a validation share, and the 2000 held-out collisions are then run twice,
-by batched DEM alone and by surrogate plus DEM fallback, with live DEM as the judge.
+by batched DEM alone and by surrogate plus DEM fallback, with live DEM deciding the true outcome.
 
 The collisions are a reduced scale of the 2000-sphere set-ups (targets of 32-512
 balls, projectiles of 1-64, 10-1000 cm/s, 40000 steps with springs softened 100-fold)
@@ -35,7 +35,7 @@ def test_surrogate_gain_and_agreement():
     r = S.evaluate(train, held, target=0.95, log=lambda m: print(m, flush=True))
     print(json.dumps(r, indent=1, default=float), flush=True)
     assert r["n_heldout"] == 2000
-    # DEM is the judge: gated agreement at least 95% on the held-out collisions
+    # measured against DEM: gated agreement at least 95% on the held-out collisions
     assert r["agreement"] >= 0.95
     # the network alone is far better than the majority class
     assert r["raw_accuracy"] >= 0.8 > max(r["heldout_classes"]) / 2000
