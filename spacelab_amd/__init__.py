@@ -744,25 +744,29 @@ class MLP:
         return float(loss.value)
 
 
-def _csv_text(fn, what: str, *args) -> str:
-    n = fn(*args, None, 0)
+def _csv_text(fn, what: str, *args, guess: int = 256) -> str:
+    """One formatting pass into `guess` bytes; a second only if the text is longer."""
+    buf = C.create_string_buffer(guess + 1)
+    n = fn(*args, buf, guess + 1)
     if n < 0:
         raise SpacelabError(int(n), what)
-    buf = C.create_string_buffer(int(n) + 1)
-    fn(*args, buf, n + 1)
+    if n > guess:
+        buf = C.create_string_buffer(int(n) + 1)
+        fn(*args, buf, n + 1)
     return buf.raw[:n].decode()
 
 
 def csv_sim_header(n: int) -> str:
     """simData.csv header line for n balls (sl_csv_sim_header)."""
-    return _csv_text(lib().sl_csv_sim_header, "sl_csv_sim_header", int(n))
+    return _csv_text(lib().sl_csv_sim_header, "sl_csv_sim_header", int(n), guess=80 * int(n) + 16)
 
 
 def csv_sim_row(pos, w, vel) -> str:
     """One simData.csv row, without its leading newline (sl_csv_sim_row)."""
     n = len(pos)
+    # a number takes at most 13 characters at 6 digits
     return _csv_text(lib().sl_csv_sim_row, "sl_csv_sim_row", n, _dp(_f64(pos, (n, 3))), _dp(_f64(w, (n, 3))),
-                     _dp(_f64(vel, (n, 3))))
+                     _dp(_f64(vel, (n, 3))), guess=154 * n + 16)
 
 
 def csv_energy_row(time: float, PE: float, KE: float, p, L, digits: int = 12) -> str:
@@ -775,12 +779,13 @@ def csv_constants(R, m, moi) -> str:
     """The constants.csv text (one "R,m,moi" line per ball)."""
     n = len(R)
     return _csv_text(lib().sl_csv_constants, "sl_csv_constants", n, _dp(_f64(R, (n,))), _dp(_f64(m, (n,))),
-                     _dp(_f64(moi, (n,))))
+                     _dp(_f64(moi, (n,))), guess=42 * n + 16)
 
 
 def csv_last_line(path: str) -> str:
     """The last line of a file as the reference's getLastLine reads it."""
-    return _csv_text(lib().sl_csv_last_line, f"sl_csv_last_line({path})", os.fsencode(path))
+    size = os.path.getsize(path) if os.path.exists(path) else 0
+    return _csv_text(lib().sl_csv_last_line, f"sl_csv_last_line({path})", os.fsencode(path), guess=size)
 
 
 def csv_parse_sim_row(row: str):
