@@ -312,8 +312,8 @@ int sl_create(sl_ctx** out, int n_max, const sl_params* p)
     };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(SL_E_HIP);
     if (hipMalloc(&c->base, sizeof(double) * (size_t)NFIELD * c->n_pad) != hipSuccess) return fail(SL_E_HIP);
-    // ordered_blocks(n) is largest for the smallest block size (8 balls), whatever n <= n_max
-    const int max_blocks = (n_max + 7) / 8 + 8;
+    // ordered_blocks(n) is largest for the smallest block size (4 balls), whatever n <= n_max
+    const int max_blocks = (n_max + 3) / 4 + 8;
     const int e_blocks = kick_blocks(n_max) + 8;
     if (hipMalloc(&c->pe_part, sizeof(double) * max_blocks) != hipSuccess) return fail(SL_E_HIP);
     if (hipMalloc(&c->e_part, sizeof(double) * 7 * e_blocks) != hipSuccess) return fail(SL_E_HIP);

@@ -268,8 +268,10 @@ hipError_t launch_copy_prev(const State& s, int n, hipStream_t st)
 
 int ordered_blocks(int n, int* bi)
 {
-    // smaller i-blocks for small systems so the grid still covers the chip
-    const int b = (n >= 4096) ? 16 : 8;
+    // the largest i-block that still gives about 940 blocks: the pass is latency-bound
+    // at a few wavefronts per SIMD, and smaller blocks raise that (measured best: 4 balls
+    // up to ~7,500 spheres, 8 up to ~15,000, 16 above; 343 vs 458 us per 5000-sphere step)
+    const int b = n >= 16 * 940 ? 16 : n >= 8 * 940 ? 8 : 4;
     if (bi) *bi = b;
     return (n + b - 1) / b;
 }
@@ -282,8 +284,10 @@ hipError_t launch_pairs_ordered(const State& s, const Phys& P, int n, int write,
     const int nb = ordered_blocks(n, &bi);
     if (bi == 16)
         k_pairs_ordered<16><<<nb, 256, 0, st>>>(s, P, n, write, pe_part, contacts);
-    else
+    else if (bi == 8)
         k_pairs_ordered<8><<<nb, 256, 0, st>>>(s, P, n, write, pe_part, contacts);
+    else
+        k_pairs_ordered<4><<<nb, 256, 0, st>>>(s, P, n, write, pe_part, contacts);
     return hipGetLastError();
 }
 
