@@ -7,11 +7,11 @@
 // component), while the expensive part -- evaluating the pair -- runs on every
 // lane of the block:
 //
-//   block = 256 threads, BI balls i, j swept in chunks of 64:
+//   block = 256 threads, BI balls i, j swept in chunks of CJ (64 for BI = 16, else 128):
 //     produce: thread (il, jq) evaluates pairs (i0+il, j0+jq+JQ*k) in the
 //              reference orientation (A = max(i,j), B = min(i,j)) and writes
 //              the signed contribution for ball i into LDS;
-//     consume: 6*BI threads, one per (ball, component), add the 64 chunk
+//     consume: 6*BI threads, one per (ball, component), add the CJ chunk
 //              contributions in ascending j into their running sums.
 //   Far pairs contribute +0.0 to aacc and j == i contributes +0.0 to both; an
 //   accumulator that starts at +0.0 under round-to-nearest never becomes -0.0,
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                        unsigned long long* __restrict__ contacts)
 {
     constexpr int NT = 256;
-    constexpr int CJ = 64;
+    constexpr int CJ = BI == 16 ? 64 : 128;   // partners per chunk (one barrier pair each)
     constexpr int JQ = NT / BI;            // threads sharing one ball i
     constexpr int CS = CJ * BI + 8;         // component stride, padded against bank conflicts
     __shared__ double cont[6 * CS];
