@@ -202,6 +202,13 @@ __device__ __forceinline__ int4 fetch_tile(const TileArgs& a, int t, bool live)
 struct JRec64 {
     double x, y, z, R, q, g;
 };
+// The mixed path's record: the binary64 geometry, and the binary32 factors it uses as
+// they are (radius in units of L0, cohesion and gravity weights), so no pair converts them.
+struct JRecMix {
+    double x, y, z, R;
+    float Rs, q, g, pad;
+};
+static_assert(sizeof(JRecMix) == sizeof(JRec64), "both records share the LDS staging area");
 
 // The pair sweep of every tile: the far-field force of each pair, with the per-pair
 // near test only in tiles the bounding-box test cannot rule out.  A near pair
@@ -244,15 +251,30 @@ __global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
     const double qi = PREC == SL_PREC_MIXED ? (double)a.aux.q32[i] : a.aux.q[i];
     const double gi = PREC == SL_PREC_MIXED ? (double)a.aux.g32[i] : a.aux.g[i];
     if (live && h == 0) {
-        JRec64 r;
-        r.x = a.px[j0];
-        r.y = a.py[j0];
-        r.z = a.pz[j0];
-        r.R = a.R[j0];
-        r.q = PREC == SL_PREC_MIXED ? (double)a.aux.q32[j0] : a.aux.q[j0];
-        r.g = PREC == SL_PREC_MIXED ? (double)a.aux.g32[j0] : a.aux.g[j0];
-        sj[tl][l] = r;
-        sj[tl][l + TB] = r;
+        if constexpr (PREC == SL_PREC_MIXED) {
+            JRecMix r;
+            r.x = a.px[j0];
+            r.y = a.py[j0];
+            r.z = a.pz[j0];
+            r.R = a.R[j0];
+            r.Rs = (float)a.R[j0] * a.L0inv;
+            r.q = a.aux.q32[j0];
+            r.g = a.aux.g32[j0];
+            r.pad = 0.f;
+            JRecMix* sm = reinterpret_cast<JRecMix*>(sj[tl]);
+            sm[l] = r;
+            sm[l + TB] = r;
+        } else {
+            JRec64 r;
+            r.x = a.px[j0];
+            r.y = a.py[j0];
+            r.z = a.pz[j0];
+            r.R = a.R[j0];
+            r.q = a.aux.q[j0];
+            r.g = a.aux.g[j0];
+            sj[tl][l] = r;
+            sj[tl][l + TB] = r;
+        }
         im[tl][l] = 0;
         jm[tl][l] = 0;
     }
@@ -271,19 +293,20 @@ __global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
             const float qif = (float)qi, gif = (float)gi, L0inv2 = a.L0inv2, L0inv = a.L0inv;
             const float r4i = 4.0f * (float)Ri * L0inv;
             float fax = 0.f, fay = 0.f, faz = 0.f, fbx = 0.f, fby = 0.f, fbz = 0.f;
+            const JRecMix* recm = reinterpret_cast<const JRecMix*>(sj[tl]) + l;
             auto pair = [&](int s, bool act, auto ntest) {
-                const JRec64& r = rec[s];
+                const JRecMix& r = recm[s];
                 const double dx = r.x - xi, dy = r.y - yi, dz = r.z - zi;
                 const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
                 const double S = Ri + r.R;
                 const double sv = __builtin_fma(-S, S, d2);
                 const float s32 = (float)sv * L0inv2;
-                const float t32 = __builtin_fmaf(r4i, (float)r.R * L0inv, s32);
+                const float t32 = __builtin_fmaf(r4i, r.Rs, s32);
                 const float st = s32 * t32;
-                float f = (qif * (float)r.q) * __builtin_amdgcn_rcpf(st * st);
+                float f = (qif * r.q) * __builtin_amdgcn_rcpf(st * st);
                 if (GRAV) {
                     const float rq = __builtin_amdgcn_rsqf((float)d2 * L0inv2);
-                    f = __builtin_fmaf(gif * (float)r.g, rq * rq * rq, f);
+                    f = __builtin_fmaf(gif * r.g, rq * rq * rq, f);
                 }
                 bool use = true;
                 if constexpr (decltype(ntest)::value) {
@@ -296,7 +319,9 @@ __global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
                         any |= 1ull << s;
                     }
                 }
-                const float dxf = (float)dx * L0inv, dyf = (float)dy * L0inv, dzf = (float)dz * L0inv;
+                // the unit scale on the factor once, not on each displacement
+                f = f * L0inv;
+                const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
                 fax = __builtin_fmaf(f, dxf, fax);
                 fay = __builtin_fmaf(f, dyf, fay);
                 faz = __builtin_fmaf(f, dzf, faz);
