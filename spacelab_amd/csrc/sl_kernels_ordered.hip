@@ -200,7 +200,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     __syncthreads();
     if (cnt) atomicAdd(&ccount, cnt);
     __syncthreads();
-    if (t == 0 && ccount) atomicAdd(contacts, (unsigned long long)ccount);
+    // contacts[0]: the running total; contacts[1]: this write step's touching pairs
+    if (t == 0 && ccount) {
+        atomicAdd(contacts, (unsigned long long)ccount);
+        if (write) atomicAdd(contacts + 1, (unsigned long long)ccount);
+    }
 }
 
 // Third pass (ColliderSingleCore.cpp:263-287): closing half kick and, on write

@@ -187,3 +187,31 @@ def test_coincident_spheres_flag_nonfinite():
         c.init_conditions()
         e = c.step(1, 1)[0]
     assert e.nonfinite
+
+
+def test_contact_count_at_write_steps():
+    """The write-step energies count the touching pairs: 150 small spheres pressed onto a
+    large one (each overlapping it by 1e-3 of the radii sum) give 150 contacts at the
+    step-0 pass and at the next write steps, in the ordered path and in the fast path."""
+    rng = np.random.default_rng(3)
+    k = 150
+    Rb, Rs = 2e-4, 1e-5
+    d = rng.standard_normal((k, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    for _ in range(300):
+        diff = d[:, None, :] - d[None, :, :]
+        dist = np.linalg.norm(diff, axis=2) + np.eye(k)
+        d += 0.002 * (diff / dist[:, :, None] ** 3).sum(1)
+        d /= np.linalg.norm(d, axis=1)[:, None]
+    pos = np.vstack([[0, 0, 0], d * (Rb + Rs) * (1 - 1e-3)])
+    R = np.array([Rb] + [Rs] * k)
+    m = 2.25 * 4 / 3 * np.pi * R ** 3
+    g = sl.BallGroup(pos, np.zeros_like(pos), np.zeros_like(pos), R, m, 0.4 * m * R * R)
+    p = sl.calibrate(sl.preset("reference"), R, m, velocity=0.36301555459799423, cor=0.64)
+    for order in (sl.ORDER_ORDERED, sl.ORDER_FAST):
+        with sl.Collider(k + 1, p.replace(order=order)) as c:
+            c.upload(g)
+            e0 = c.init_conditions()
+            ew = c.step(3, 1)
+        assert e0.contacts == k, order
+        assert len(ew) == 3 and all(e.contacts == k for e in ew), (order, [e.contacts for e in ew])
