@@ -566,7 +566,7 @@ struct BallArgs {
 constexpr int KB_W = 8;              // wavefronts per k_ball block (one 64-ball block)
 constexpr int KB_T = KB_W * TB;
 constexpr int BPB = TB;              // balls per k_ball block
-constexpr int SLOTS_IN_FLIGHT = 8;   // slots per lane loaded together
+constexpr int SLOTS_IN_FLIGHT = 10;  // slots per lane loaded together: 40 loads, one batch for systems of up to 80 blocks
 constexpr int KB_MC = 4;             // partner masks kept per (wavefront, ball)
 
 // One 64-ball block per workgroup; lane l of every wavefront works for ball l.
