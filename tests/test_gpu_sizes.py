@@ -4,8 +4,8 @@
   block, one over, and two blocks plus one: the fast path pads every system to whole
   64-sphere blocks, so these put padded lanes in the diagonal tile, in a full tile and
   in a tile of one real sphere.
-- 7,519 and 7,520 spheres: the ordered pair pass takes 4 spheres per workgroup below
-  7,520 and 8 from there on.
+- 7,519, 7,520 and 15,040 spheres: the ordered pair pass takes 4 spheres per workgroup
+  below 7,520, 8 from there on and 16 from 15,040.
 
 The ordered path is bitwise the oracle at every size.  The fast path meets its one-step
 bars (forces and torques to 1e-13 relative, velocities to 1e-14, positions to 1e-15) and,
@@ -55,10 +55,10 @@ def _pe_close(e, ref_pe, rtol):
     assert abs(e.PE - ref_pe) <= rtol * abs(ref_pe) + 1e-300, (e.PE, ref_pe)
 
 
-@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 129, 7519, 7520])
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 129, 7519, 7520, 15040])
 def test_ordered_sizes_bitwise(oracle_mod, n):
     g, p = _case(n)
-    steps = 20 if n < 1000 else 3
+    steps = 20 if n < 1000 else 3 if n < 10000 else 2
     print(f"ordered n={n} steps={steps}", flush=True)
     out, e0, ew = _run(g, p, steps, steps)
     st = oracle_mod.State(g.pos, g.vel, g.w, g.R, g.m, g.moi)
