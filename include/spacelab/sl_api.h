@@ -129,9 +129,14 @@ void sl_destroy(sl_ctx* ctx);
  *                      built-in autotuner on first use, or 1, 2, 4.  The split changes
  *                      the summation order inside a tile, so runs compare bitwise only
  *                      at equal split.  sl_get_option reports the split in use (0
- *                      before the first fast step). */
+ *                      before the first fast step).
+ *   SL_OPT_BALL_WAVES  wavefronts per workgroup of the per-ball kernel: 0 (default) = 8
+ *                      when the state has at most one 64-ball block per compute unit,
+ *                      else 4; or 4, 8.  Results are identical either way.
+ *                      sl_get_option reports the shape the next step uses. */
 #define SL_OPT_GRAPHS 1
 #define SL_OPT_TILE_SPLIT 2
+#define SL_OPT_BALL_WAVES 3
 int sl_set_option(sl_ctx* ctx, int option, int value);
 int sl_get_option(sl_ctx* ctx, int option, int* value);
 

@@ -28,8 +28,8 @@ LIB_PATH = os.path.join(_PKG, "lib", "libspacelab.so")
 
 PREC_FP64, PREC_MIXED = 0, 1
 ORDER_ORDERED, ORDER_FAST = 0, 1
-OPT_GRAPHS, OPT_TILE_SPLIT = 1, 2
-_OPTS = {"graphs": OPT_GRAPHS, "tile_split": OPT_TILE_SPLIT}
+OPT_GRAPHS, OPT_TILE_SPLIT, OPT_BALL_WAVES = 1, 2, 3
+_OPTS = {"graphs": OPT_GRAPHS, "tile_split": OPT_TILE_SPLIT, "ball_waves": OPT_BALL_WAVES}
 
 ERRORS = {-1: "SL_E_ARG", -2: "SL_E_HIP", -3: "SL_E_STATE", -4: "SL_E_CAPACITY", -5: "SL_E_NOGPU"}
 
@@ -519,7 +519,8 @@ class Collider:
         self._check(lib().sl_synchronize(self._ctx), "sl_synchronize")
 
     def set_option(self, name: str, value: int):
-        """'graphs' (0/1) or 'tile_split' (0 = autotune, 1, 2, 4) of the fast engine."""
+        """'graphs' (0/1), 'tile_split' (0 = autotune, 1, 2, 4) or 'ball_waves' (0 = by
+        size, 4, 8) of the fast engine."""
         self._check(lib().sl_set_option(self._ctx, _OPTS[name], int(value)), "sl_set_option")
 
     def get_option(self, name: str) -> int:

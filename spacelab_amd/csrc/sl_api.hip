@@ -302,6 +302,7 @@ int sl_create(sl_ctx** out, int n_max, const sl_params* p)
         return SL_E_NOGPU;
     }
     c->p = *p;
+    c->fb.n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     c->n_max = n_max;
     c->n_pad = (n_max + 255) / 256 * 256;
     auto fail = [&](int code) {
@@ -368,6 +369,12 @@ int sl_set_option(sl_ctx* c, int option, int value)
         c->fb.split = 0;
         fast_clear_graphs(c->fb);
         return SL_OK;
+    case SL_OPT_BALL_WAVES:
+        if (value != 0 && value != 4 && value != 8)
+            return set_err(c, SL_E_ARG, "ball waves must be 0 (by size), 4 or 8 (got %d)", value);
+        c->fb.forced_ball_waves = value;
+        fast_clear_graphs(c->fb);
+        return SL_OK;
     default:
         return set_err(c, SL_E_ARG, "unknown option %d", option);
     }
@@ -382,6 +389,9 @@ int sl_get_option(sl_ctx* c, int option, int* value)
         return SL_OK;
     case SL_OPT_TILE_SPLIT:
         *value = c->fb.split;
+        return SL_OK;
+    case SL_OPT_BALL_WAVES:
+        *value = fast_ball_waves(c->fb);
         return SL_OK;
     default:
         return set_err(c, SL_E_ARG, "unknown option %d", option);

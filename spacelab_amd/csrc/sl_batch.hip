@@ -107,6 +107,7 @@ int sl_batch_create(sl_batch** out, int max_systems, int max_balls, const sl_par
         return SL_E_NOGPU;
     }
     b->p = *p;
+    b->fb.n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     b->max_sys = max_systems;
     b->max_balls = max_balls;
     // every system starts on a 64-slot boundary: at most 63 padding slots each
@@ -263,6 +264,12 @@ int sl_batch_set_option(sl_batch* b, int option, int value)
         b->fb.split = 0;
         fast_clear_graphs(b->fb);
         return SL_OK;
+    case SL_OPT_BALL_WAVES:
+        if (value != 0 && value != 4 && value != 8)
+            return berr(b, SL_E_ARG, "ball waves must be 0 (by size), 4 or 8 (got %d)", value);
+        b->fb.forced_ball_waves = value;
+        fast_clear_graphs(b->fb);
+        return SL_OK;
     default:
         return berr(b, SL_E_ARG, "unknown option %d", option);
     }
@@ -277,6 +284,9 @@ int sl_batch_get_option(sl_batch* b, int option, int* value)
         return SL_OK;
     case SL_OPT_TILE_SPLIT:
         *value = b->fb.split;
+        return SL_OK;
+    case SL_OPT_BALL_WAVES:
+        *value = fast_ball_waves(b->fb);
         return SL_OK;
     default:
         return berr(b, SL_E_ARG, "unknown option %d", option);

@@ -20,17 +20,19 @@
 //              to slot J of the balls of block I and slot I of the balls of block J
 //              (system-local block numbers), so every (ball, slot) is written by
 //              exactly one tile per step and nothing needs clearing.
-//   k_ball     one workgroup per 64-ball block, eight wavefronts with lane l
-//              working for ball l: each wavefront sums the far-field partials of
-//              every eighth slot (Kahan-compensated in the mixed mode) and counts the
-//              ball's partner-mask bits; the block numbers its near pairs ball by
-//              ball and evaluates them with the reference arithmetic, one per thread,
-//              in rounds of up to 512; then wavefront 0 adds the near results of
+//   k_ball     one workgroup per 64-ball block, eight or four wavefronts with lane
+//              l working for ball l: the far-field partials of every eighth slot
+//              (one slot class) are summed by one wavefront (Kahan-compensated in
+//              the mixed mode), which counts the ball's partner-mask bits; the block
+//              numbers its near pairs ball by ball and evaluates them with the
+//              reference arithmetic, one per thread, in rounds of one entry per
+//              thread; then wavefront 0 adds the near results of
 //              each ball in entry order, forms acc and aacc, applies the closing
 //              kick, the write-step energies and the next step's drift (writing the
 //              other state set).  A near pair is evaluated once for each of its two
 //              balls, by the blocks that hold them.
-// Every sum has a fixed order, so results are reproducible run to run.
+// Every sum has a fixed order at a given tile split, so results are reproducible run
+// to run; the k_ball shape does not change them.
 #pragma once
 
 #include "sl_fast_pair.hpp"
@@ -563,36 +565,46 @@ struct BallArgs {
     int* flags;         // [1]: a non-finite state appeared
 };
 
-constexpr int KB_W = 8;              // wavefronts per k_ball block (one 64-ball block)
-constexpr int KB_T = KB_W * TB;
 constexpr int BPB = TB;              // balls per k_ball block
+constexpr int KB_C = 8;              // slot classes K = c, c + KB_C, ...: they fix the order of every sum
 constexpr int SLOTS_IN_FLIGHT = 10;  // slots per lane loaded together: 40 loads, one batch for systems of up to 80 blocks
-constexpr int KB_MC = 4;             // partner masks kept per (wavefront, ball)
+// k_ball workgroups have 8 wavefronts (one slot class each: the fewest dependent loads
+// per wavefront, for a lone system of few blocks) or 4 (two classes each, 161 VGPRs and
+// 48 KB of LDS: three workgroups share a CU, for launches of more blocks than CUs).
+// The results are the same bit for bit.
+constexpr int kb_threads(int nwv) { return nwv * TB; }
 
-// One 64-ball block per workgroup; lane l of every wavefront works for ball l.
+// One 64-ball block per workgroup; lane l of every wavefront works for ball l.  The
+// slots of the block fall into KB_C classes, K = c, c + KB_C, ...; wavefront w takes the
+// classes c = w, w + NWV, ...
 //
-// 1. Wavefront w sums the far-field partials of the slots K = w, w + 8, ... in order
+// 1. For each of its classes a wavefront sums the far-field partials in slot order
 //    (reads of 64 consecutive doubles; Kahan-compensated in the mixed mode), counts
-//    the near-partner bits of those slots and keeps the first KB_MC nonzero partner
-//    masks of each ball in LDS (more than that: they are read again).
-// 2. The block's near entries are numbered ball by ball (then by wavefront, slot and
-//    bit), and all 512 threads evaluate them, KB_T at a time, with the reference
+//    the near-partner bits and keeps the first MC nonzero partner masks of each ball
+//    in LDS (more than that: they are read again).
+// 2. The block's near entries are numbered ball by ball (then by class, slot and bit),
+//    and all threads evaluate them, one round of NWV * 64 at a time, with the reference
 //    arithmetic and orientation.  Lane l of wavefront 0 adds its ball's results in
 //    entry order.  Each near pair is so evaluated once from each of its balls,
 //    identically, and no sum needs an atomic.
-// 3. Wavefront 0 combines the eight far-field sums in a fixed order and finishes the
-//    step for the 64 balls (coalesced stores).
-template <bool COMP, bool WRITE>
-__global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
+// 3. Wavefront 0 combines the KB_C class sums in a fixed order and finishes the step
+//    for the 64 balls (coalesced stores).
+template <bool COMP, bool WRITE, int NWV>
+__global__ __launch_bounds__(kb_threads(NWV)) void k_ball(BallArgs a)
 {
-    __shared__ double sf[KB_W][3][TB];       // far-field force sums per wavefront
-    __shared__ int s_cnt[KB_W][TB];          // near entries per (wavefront, ball)
-    __shared__ int s_off[KB_W][TB];          // first entry of (wavefront, ball)
+    static_assert(NWV == 8 || NWV == 4, "8 or 4 wavefronts");
+    constexpr int KB_W = NWV;                 // wavefronts
+    constexpr int KB_T = kb_threads(NWV);     // threads, and near entries per round
+    constexpr int KB_MC = NWV == 8 ? 4 : 2;   // partner masks kept per (class, ball)
+    __shared__ double sf[KB_C][3][TB];       // far-field force sums per class
+    __shared__ int s_cnt[KB_C][TB];          // near entries per (class, ball)
+    __shared__ int s_off[KB_C][TB];          // first entry of (class, ball)
+    __shared__ int s_nm[KB_C][TB];           // slots of (class, ball) with near partners
     __shared__ int s_total;
     __shared__ int ent[KB_T];                // entries of the current round
     __shared__ NearRes res[KB_T];
-    __shared__ unsigned long long s_mk[KB_W][KB_MC][TB];   // nonzero partner masks ...
-    __shared__ int s_mks[KB_W][KB_MC][TB];                  // ... and their slots
+    __shared__ unsigned long long s_mk[KB_C][KB_MC][TB];   // nonzero partner masks ...
+    __shared__ int s_mks[KB_C][KB_MC][TB];                  // ... and their slots
     const int w = threadIdx.x / TB, l = threadIdx.x % TB;
     const int i = blockIdx.x * TB + l;
     const int sys = a.blk_sys[blockIdx.x];
@@ -622,62 +634,66 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
         Rb = c.f[RAD][i];
     }
 
-    double fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
-    int cn = 0;                    // near entries of this (wavefront, ball)
-    int nm = 0;                    // slots of this (wavefront, ball) with near partners
-    for (int K0 = w; K0 < nblk; K0 += SLOTS_IN_FLIGHT * KB_W) {
-        double q[SLOTS_IN_FLIGHT][3];
-        unsigned long long mk[SLOTS_IN_FLIGHT];
+#pragma unroll 1
+    for (int cl = w; cl < KB_C; cl += KB_W) {
+        double fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
+        int cn = 0;                    // near entries of this (class, ball)
+        int nm = 0;                    // slots of this (class, ball) with near partners
+        for (int K0 = cl; K0 < nblk; K0 += SLOTS_IN_FLIGHT * KB_C) {
+            double q[SLOTS_IN_FLIGHT][3];
+            unsigned long long mk[SLOTS_IN_FLIGHT];
 #pragma unroll
-        for (int u = 0; u < SLOTS_IN_FLIGHT; u++) {
-            const int K = K0 + u * KB_W;
-            const int Kc = K < nblk ? K : 0;
-            const double* pp = a.part + (size_t)Kc * 3 * np + ic;
-            q[u][0] = pp[0];
-            q[u][1] = pp[np];
-            q[u][2] = pp[2 * np];
-            mk[u] = K < nblk ? a.nbm[(size_t)Kc * np + ic] : 0ull;
-        }
+            for (int u = 0; u < SLOTS_IN_FLIGHT; u++) {
+                const int K = K0 + u * KB_C;
+                const int Kc = K < nblk ? K : 0;
+                const double* pp = a.part + (size_t)Kc * 3 * np + ic;
+                q[u][0] = pp[0];
+                q[u][1] = pp[np];
+                q[u][2] = pp[2 * np];
+                mk[u] = K < nblk ? a.nbm[(size_t)Kc * np + ic] : 0ull;
+            }
 #pragma unroll
-        for (int u = 0; u < SLOTS_IN_FLIGHT; u++) {
-            if (K0 + u * KB_W >= nblk) break;
-            cn += __popcll(mk[u]);
-            if (mk[u]) {
-                if (nm < KB_MC) {
-                    s_mk[w][nm][l] = mk[u];
-                    s_mks[w][nm][l] = K0 + u * KB_W;
+            for (int u = 0; u < SLOTS_IN_FLIGHT; u++) {
+                if (K0 + u * KB_C >= nblk) break;
+                cn += __popcll(mk[u]);
+                if (mk[u]) {
+                    if (nm < KB_MC) {
+                        s_mk[cl][nm][l] = mk[u];
+                        s_mks[cl][nm][l] = K0 + u * KB_C;
+                    }
+                    nm++;
                 }
-                nm++;
-            }
-            if (COMP) {
-                double y, t;
-                y = q[u][0] - cx; t = fx + y; cx = (t - fx) - y; fx = t;
-                y = q[u][1] - cy; t = fy + y; cy = (t - fy) - y; fy = t;
-                y = q[u][2] - cz; t = fz + y; cz = (t - fz) - y; fz = t;
-            } else {
-                fx = fx + q[u][0];
-                fy = fy + q[u][1];
-                fz = fz + q[u][2];
+                if (COMP) {
+                    double y, t;
+                    y = q[u][0] - cx; t = fx + y; cx = (t - fx) - y; fx = t;
+                    y = q[u][1] - cy; t = fy + y; cy = (t - fy) - y; fy = t;
+                    y = q[u][2] - cz; t = fz + y; cz = (t - fz) - y; fz = t;
+                } else {
+                    fx = fx + q[u][0];
+                    fy = fy + q[u][1];
+                    fz = fz + q[u][2];
+                }
             }
         }
+        sf[cl][0][l] = fx - cx;
+        sf[cl][1][l] = fy - cy;
+        sf[cl][2][l] = fz - cz;
+        s_cnt[cl][l] = cn;
+        s_nm[cl][l] = nm;
     }
-    sf[w][0][l] = fx - cx;
-    sf[w][1][l] = fy - cy;
-    sf[w][2][l] = fz - cz;
-    s_cnt[w][l] = cn;
     __syncthreads();
 
-    // number the block's near entries: ball-major, then wavefront, slot and bit
+    // number the block's near entries: ball-major, then class, slot and bit
     if (w == 0) {
         int tot = 0;
-        for (int q = 0; q < KB_W; q++) tot += s_cnt[q][l];
+        for (int q = 0; q < KB_C; q++) tot += s_cnt[q][l];
         int incl = tot;
         for (int m = 1; m < TB; m <<= 1) {
             const int y = __shfl_up(incl, m, TB);
             if (l >= m) incl += y;
         }
         int o = incl - tot;
-        for (int q = 0; q < KB_W; q++) {
+        for (int q = 0; q < KB_C; q++) {
             s_off[q][l] = o;
             o += s_cnt[q][l];
         }
@@ -689,24 +705,28 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
     double n[6] = {0, 0, 0, 0, 0, 0};
     double npe = 0;
     int ncon = 0, npair = 0;
-    const int my0 = s_off[w][l];   // (wavefront, ball) entries are [my0, my0 + cn)
     for (int r0 = 0; r0 < total; r0 += KB_T) {
         const int r1 = r0 + KB_T;
-        // write this round's entries of (wavefront w, ball l)
-        if (cn && my0 < r1 && my0 + cn > r0) {
-            int k = my0;
-            auto put = [&](int K, unsigned long long m) {
-                while (m && k < r1) {
-                    const int bit = __ffsll((long long)m) - 1;
-                    m &= m - 1;
-                    if (k >= r0) ent[k - r0] = l | (bit << 6) | (K << 12);
-                    k++;
+        // write this round's entries of (class cl, ball l) for the classes of wavefront w
+#pragma unroll 1
+        for (int cl = w; cl < KB_C; cl += KB_W) {
+            const int cn = s_cnt[cl][l], nm = s_nm[cl][l];
+            const int my0 = s_off[cl][l];   // (class, ball) entries are [my0, my0 + cn)
+            if (cn && my0 < r1 && my0 + cn > r0) {
+                int k = my0;
+                auto put = [&](int K, unsigned long long m) {
+                    while (m && k < r1) {
+                        const int bit = __ffsll((long long)m) - 1;
+                        m &= m - 1;
+                        if (k >= r0) ent[k - r0] = l | (bit << 6) | (K << 12);
+                        k++;
+                    }
+                };
+                if (nm <= KB_MC) {
+                    for (int q = 0; q < nm && k < r1; q++) put(s_mks[cl][q][l], s_mk[cl][q][l]);
+                } else {
+                    for (int K = cl; K < nblk && k < r1; K += KB_C) put(K, a.nbm[(size_t)K * np + ic]);
                 }
-            };
-            if (nm <= KB_MC) {
-                for (int q = 0; q < nm && k < r1; q++) put(s_mks[w][q][l], s_mk[w][q][l]);
-            } else {
-                for (int K = w; K < nblk && k < r1; K += KB_W) put(K, a.nbm[(size_t)K * np + ic]);
             }
         }
         __syncthreads();
@@ -722,7 +742,7 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
         if (w == 0) {
             // ball l's entries in this round, in order
             int tot = 0;
-            for (int q = 0; q < KB_W; q++) tot += s_cnt[q][l];
+            for (int q = 0; q < KB_C; q++) tot += s_cnt[q][l];
             const int b0 = s_off[0][l], b1 = b0 + tot;
             const int k0 = b0 > r0 ? b0 : r0, k1 = b1 < r1 ? b1 : r1;
             for (int k = k0; k < k1; k++) {
@@ -741,7 +761,7 @@ __global__ __launch_bounds__(KB_T) void k_ball(BallArgs a)
     double F[3], T[3];
     for (int k = 0; k < 3; k++) {
         double s = sf[0][k][l], cmp = 0;
-        for (int q = 1; q < KB_W; q++) {
+        for (int q = 1; q < KB_C; q++) {
             if (COMP) {
                 const double y = sf[q][k][l] - cmp, t = s + y;
                 cmp = (t - s) - y;

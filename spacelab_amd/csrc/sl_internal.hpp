@@ -98,6 +98,8 @@ struct FastBuffers {
     int split = 0;                // wavefronts per 64x64 tile (0: not tuned yet)
     int split_key = -1;           // (precision, gravity) the choice was tuned for
     int forced_split = 0;         // 1, 2 or 4 forces the choice
+    int n_cu = 256;               // compute units of the device (k_ball's shape)
+    int forced_ball_waves = 0;    // 4 or 8 forces k_ball's wavefronts per workgroup
     bool use_graphs = true;
     LaunchHook hook{};
     struct GraphKey {
@@ -132,6 +134,8 @@ int fast_steps(FastBuffers& fb, State& canon, int n_steps, long long* step_count
                sl_energy* e_writes, int cap, hipStream_t st, std::string* err);
 int fast_max_tile_blocks(int n_pad);
 void fast_clear_graphs(FastBuffers& fb);
+// wavefronts per k_ball workgroup for the current layout (forced, or by block count)
+int fast_ball_waves(const FastBuffers& fb);
 hipError_t fast_reset_counters(FastBuffers& fb, hipStream_t st);
 int fast_read_counters(FastBuffers& fb, hipStream_t st, long long* contacts, long long* near_pairs);
 

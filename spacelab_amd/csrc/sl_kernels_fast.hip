@@ -242,10 +242,16 @@ int launch_ball_step(FastBuffers& fb, bool write, bool kick, bool drift_next, hi
     a.flags = fb.flags;
     const int blocks = fb.lay.n_used / TB;
     if (fb.hook.before) fb.hook.before(fb.hook.user, 1);
-    if (fb.precision == SL_PREC_MIXED)
-        write ? k_ball<true, true><<<blocks, KB_T, 0, st>>>(a) : k_ball<true, false><<<blocks, KB_T, 0, st>>>(a);
-    else
-        write ? k_ball<false, true><<<blocks, KB_T, 0, st>>>(a) : k_ball<false, false><<<blocks, KB_T, 0, st>>>(a);
+    const bool mixed = fb.precision == SL_PREC_MIXED;
+    if (fast_ball_waves(fb) == 8) {
+        constexpr int T = kb_threads(8);
+        if (mixed) write ? k_ball<true, true, 8><<<blocks, T, 0, st>>>(a) : k_ball<true, false, 8><<<blocks, T, 0, st>>>(a);
+        else write ? k_ball<false, true, 8><<<blocks, T, 0, st>>>(a) : k_ball<false, false, 8><<<blocks, T, 0, st>>>(a);
+    } else {
+        constexpr int T = kb_threads(4);
+        if (mixed) write ? k_ball<true, true, 4><<<blocks, T, 0, st>>>(a) : k_ball<true, false, 4><<<blocks, T, 0, st>>>(a);
+        else write ? k_ball<false, true, 4><<<blocks, T, 0, st>>>(a) : k_ball<false, false, 4><<<blocks, T, 0, st>>>(a);
+    }
     if (fb.hook.after) fb.hook.after(fb.hook.user, 1);
     if (hipGetLastError() != hipSuccess) return err_hip(err, "ball kernel launch");
     fb.cur = 1 - fb.cur;
@@ -423,6 +429,15 @@ int ensure_engine(FastBuffers& fb, State& canon, hipStream_t st, std::string* er
 }
 
 }  // namespace
+
+// Up to one 64-ball block per CU: eight wavefronts, each with one slot class to load.
+// More blocks: four wavefronts, so that three workgroups share a CU.  The results are
+// the same bit for bit.
+int fast_ball_waves(const FastBuffers& fb)
+{
+    if (fb.forced_ball_waves == 4 || fb.forced_ball_waves == 8) return fb.forced_ball_waves;
+    return fb.lay.n_used / TB <= fb.n_cu ? 8 : 4;
+}
 
 FastLayout fast_layout(int n_sys, const int* n_per_sys)
 {

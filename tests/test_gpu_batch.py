@@ -203,3 +203,84 @@ def test_batch_errors():
         with pytest.raises(sl.SpacelabError) as ei:
             b.set_option("tile_split", 3)
         assert ei.value.code == -1
+        with pytest.raises(sl.SpacelabError) as ei:
+            b.set_option("ball_waves", 6)
+        assert ei.value.code == -1
+
+
+def _batch_run(groups, params, steps, we, ball_waves):
+    with sl.Batch(len(groups), sum(g.num_particles for g in groups), params[0]) as b:
+        b.set_option("tile_split", 2)
+        b.set_option("ball_waves", ball_waves)
+        b.upload(groups, kin=[p.kin for p in params], kout=[p.kout for p in params],
+                 dt=[p.dt for p in params])
+        e0 = b.init_conditions()
+        ew = b.step(steps, we)
+        return b.download(), e0, ew, b.get_option("ball_waves")
+
+
+def test_ball_kernel_shapes_agree_bitwise():
+    """The per-ball kernel in workgroups of 4 wavefronts (two slot classes each) and of
+    8 (one each) gives the same bits: the classes fix every sum's order.  A batch of 300
+    systems (more 64-ball blocks than compute units: 4 by default) against the same
+    batch forced to 8, and a lone system (8 by default) forced to 4, with write steps."""
+    sizes = [40 + (37 * k) % 160 for k in range(300)]
+    groups, params = _ensemble(sizes, 0x5EED0600)
+    a, ea0, eaw, wa = _batch_run(groups, params, 60, 30, 0)
+    b, eb0, ebw, wb = _batch_run(groups, params, 60, 30, 8)
+    assert (wa, wb) == (4, 8)
+    for k in range(len(sizes)):
+        for f in FIELDS:
+            assert bits_equal(getattr(a[k], f), getattr(b[k], f)), (k, f)
+        assert _ebytes(ea0[k]) == _ebytes(eb0[k]), k
+        for w in range(len(eaw)):
+            assert _ebytes(eaw[w][k]) == _ebytes(ebw[w][k]), (k, w)
+    g = sl.generate_bpca(2000, seed=0x5EED0601, v_sigma=2.0)
+    p = _params(g)
+    outs = []
+    for waves in (0, 4):
+        with sl.Collider(g.num_particles, p) as c:
+            c.set_option("tile_split", 2)
+            c.set_option("ball_waves", waves)
+            c.upload(g)
+            e0 = c.init_conditions()
+            ew = c.step(60, 30)
+            outs.append((c.download(), e0, ew, c.get_option("ball_waves")))
+    assert outs[0][3] == 8 and outs[1][3] == 4
+    for f in FIELDS:
+        assert bits_equal(getattr(outs[0][0], f), getattr(outs[1][0], f)), f
+    assert _ebytes(outs[0][1]) == _ebytes(outs[1][1])
+    assert [_ebytes(e) for e in outs[0][2]] == [_ebytes(e) for e in outs[1][2]]
+
+
+def test_ball_kernel_shapes_agree_with_many_partners():
+    """One sphere pressed by 4000 small ones (a Fibonacci lattice over its surface): its
+    block numbers more than 4,000 near entries, reading partner masks again, and
+    evaluates them over at least 8 rounds of 512 or 16 of 256.  Both kernel shapes give
+    the same bits."""
+    k = 4000
+    rs, Rb = 1e-5, 4e-4
+    t = np.arange(k) + 0.5
+    phi, th = np.arccos(1 - 2 * t / k), np.pi * (1 + 5 ** 0.5) * t
+    d = np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], 1)
+    pos = np.vstack([[0.0, 0.0, 0.0], d * (Rb + rs) * (1 - 1e-3)])
+    R = np.array([Rb] + [rs] * k)
+    m = 2.25 * 4 / 3 * np.pi * R ** 3
+    rng = np.random.default_rng(12)
+    g = sl.BallGroup(pos, rng.standard_normal(pos.shape) * 1e-3, rng.standard_normal(pos.shape) * 1e-2,
+                     R, m, 0.4 * m * R * R)
+    p = sl.calibrate(sl.preset("config"), R, m, velocity=V_CUSTOM, cor=0.64).replace(order=sl.ORDER_FAST)
+    outs = []
+    for waves in (8, 4):
+        with sl.Collider(k + 1, p) as c:
+            c.set_option("tile_split", 2)
+            c.set_option("ball_waves", waves)
+            c.upload(g)
+            e0 = c.init_conditions()
+            ew = c.step(4, 2)
+            outs.append((c.download(), e0, ew))
+    assert outs[0][1].contacts == k
+    for f in FIELDS:
+        assert bits_equal(getattr(outs[0][0], f), getattr(outs[1][0], f)), f
+    assert _ebytes(outs[0][1]) == _ebytes(outs[1][1])
+    assert [_ebytes(e) for e in outs[0][2]] == [_ebytes(e) for e in outs[1][2]]
