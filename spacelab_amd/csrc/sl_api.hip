@@ -232,7 +232,8 @@ int pair_pass(sl_ctx* c, bool write, bool kick, int* n_pe_blocks)
         ProfScope ps(c, 0);
         HIPCHK(c, launch_pairs_ordered(c->st, P, c->n, write ? 1 : 0, c->pe_part, c->cnt, c->stream));
     }
-    *n_pe_blocks = ordered_blocks(c->n, nullptr);
+    // one sphere: no pair pass ran and no PE partial was written
+    *n_pe_blocks = c->n > 1 ? ordered_blocks(c->n, nullptr) : 0;
     {
         ProfScope ps(c, 1);
         HIPCHK(c, launch_kick(c->st, c->n, c->p.dt, write ? 1 : 0, kick ? 1 : 0, c->e_part, c->nonfinite, c->stream));
