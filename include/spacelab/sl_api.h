@@ -130,9 +130,10 @@ void sl_destroy(sl_ctx* ctx);
  *                      the summation order inside a tile, so runs compare bitwise only
  *                      at equal split.  sl_get_option reports the split in use (0
  *                      before the first fast step).
- *   SL_OPT_BALL_WAVES  wavefronts per workgroup of the per-ball kernel: 0 (default) = 8
- *                      when the state has at most one 64-ball block per compute unit,
- *                      else 4; or 4, 8.  Results are identical either way.
+ *   SL_OPT_BALL_WAVES  wavefronts per workgroup of the per-ball kernel: 0 (default) = 4
+ *                      when the state has more 64-ball blocks than the device has
+ *                      compute units and no system has more than 80 blocks (many small
+ *                      systems), else 8; or 4, 8.  Results are identical either way.
  *                      sl_get_option reports the shape the next step uses. */
 #define SL_OPT_GRAPHS 1
 #define SL_OPT_TILE_SPLIT 2

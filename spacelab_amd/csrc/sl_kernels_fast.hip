@@ -430,13 +430,15 @@ int ensure_engine(FastBuffers& fb, State& canon, hipStream_t st, std::string* er
 
 }  // namespace
 
-// Up to one 64-ball block per CU: eight wavefronts, each with one slot class to load.
-// More blocks: four wavefronts, so that three workgroups share a CU.  The results are
-// the same bit for bit.
+// Four wavefronts (three workgroups per CU) when there are more 64-ball blocks than CUs
+// and every system has at most 80 blocks: many small systems, as in a collision
+// ensemble, where the block count, not the slots per ball, sets the time.  Otherwise
+// eight: a lone system, or large ones, whose balls have many slots to load and more
+// partner masks to keep.  The results are the same bit for bit.
 int fast_ball_waves(const FastBuffers& fb)
 {
     if (fb.forced_ball_waves == 4 || fb.forced_ball_waves == 8) return fb.forced_ball_waves;
-    return fb.lay.n_used / TB <= fb.n_cu ? 8 : 4;
+    return fb.lay.n_used / TB > fb.n_cu && fb.lay.nb_max <= 80 ? 4 : 8;
 }
 
 FastLayout fast_layout(int n_sys, const int* n_per_sys)
