@@ -108,3 +108,20 @@ def test_bad_params_rejected_before_device():
     with pytest.raises(sl.SpacelabError) as ei:
         sl.Collider(16, p)
     assert ei.value.code in (-1, -5)
+
+
+def test_binding_constants_match_header():
+    """The Python binding's codes are the header's #defines (precision, order, options,
+    error codes, outcome classes, descriptor count)."""
+    txt = open(os.path.join(ROOT, "include", "spacelab", "sl_api.h")).read()
+    defs = {m.group(1): int(m.group(2)) for m in re.finditer(r"^#define (SL_\w+) (-?\d+)", txt, flags=re.M)}
+    assert defs["SL_PREC_FP64"] == sl.PREC_FP64 and defs["SL_PREC_MIXED"] == sl.PREC_MIXED
+    assert defs["SL_ORDER_ORDERED"] == sl.ORDER_ORDERED and defs["SL_ORDER_FAST"] == sl.ORDER_FAST
+    assert defs["SL_OPT_GRAPHS"] == sl.OPT_GRAPHS
+    assert defs["SL_OPT_TILE_SPLIT"] == sl.OPT_TILE_SPLIT
+    assert defs["SL_OPT_BALL_WAVES"] == sl.OPT_BALL_WAVES
+    for code, name in sl.ERRORS.items():
+        assert defs[name] == code, name
+    assert (defs["SL_OUTCOME_STICK"], defs["SL_OUTCOME_BOUNCE"], defs["SL_OUTCOME_FRAGMENT"]) == (
+        sl.OUTCOME_STICK, sl.OUTCOME_BOUNCE, sl.OUTCOME_FRAGMENT)
+    assert defs["SL_N_DESCRIPTORS"] == sl.N_DESCRIPTORS
