@@ -204,10 +204,11 @@ def ensemble(n_coll=512, steps=200, warmup=100, horizon=50000, seed=0x5EED0003):
             t0 = time.perf_counter()
             ew = b.step(steps, steps)           # energies of every collision at the last step
             b.download()
-            return e0, ew_warm[-1], ew[-1], time.perf_counter() - t0, b.get_option("tile_split")
+            return (e0, ew_warm[-1], ew[-1], time.perf_counter() - t0, b.get_option("tile_split"),
+                    b.get_option("ball_waves"))
 
-    e0, w64, e64, sec, split = run(sl.PREC_FP64)
-    _, wmx, emx, sec_mx, _ = run(sl.PREC_MIXED)
+    e0, w64, e64, sec, split, waves = run(sl.PREC_FP64)
+    _, wmx, emx, sec_mx, _, _ = run(sl.PREC_MIXED)
 
     # per collision: |E_mixed - E_fp64| / (|PE| + KE) after the same steps from the same state
     def rel_of(xs, ys):
@@ -218,7 +219,7 @@ def ensemble(n_coll=512, steps=200, warmup=100, horizon=50000, seed=0x5EED0003):
     return {"collisions": n_coll, "balls": int(sizes.sum()), "balls_min_max": [int(sizes.min()), int(sizes.max())],
             "pairs_per_step": pairs, "timed_steps": steps, "ms_per_step": 1e3 * step_s,
             "pairs_per_s": pairs / step_s, "horizon_steps": horizon,
-            "collisions_per_s": n_coll / (horizon * step_s), "tile_split": split,
+            "collisions_per_s": n_coll / (horizon * step_s), "tile_split": split, "ball_waves": waves,
             "mixed_ms_per_step": 1e3 * sec_mx / steps,
             "mixed_vs_fp64_energy_rel": {
                 f"after_{warmup}_steps": rel_of(wmx, w64), f"after_{warmup + steps}_steps": rel_of(emx, e64),
