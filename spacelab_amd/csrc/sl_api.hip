@@ -438,13 +438,19 @@ int sl_upload(sl_ctx* c, int n, const double* pos3, const double* vel3, const do
         h[(size_t)MOI * np + i] = moi[i];
     }
     // padding balls: unit radius and mass far away are never read (loops stop at n)
+    // the context counts as uploaded only once the fast engine is configured for the new
+    // state: a failed configuration leaves it unusable rather than half set up
+    c->uploaded = false;
+    c->forces_valid = false;
     HIPCHK(c, hipMemcpyAsync(c->base, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->n = n;
     c->step = 0;
+    const int rc = configure_fast(c);
+    if (rc) return rc;
     c->uploaded = true;
     c->forces_valid = (acc3 != nullptr);
-    return configure_fast(c);
+    return SL_OK;
 }
 
 int sl_init_forces(sl_ctx* c, sl_energy* e)

@@ -20,11 +20,17 @@ namespace sl {
 // and one Newton-Raphson step y <- y + y (1 - x y), 48.7 good bits (2.2e-15 relative):
 // the size of the far-field algebra's other reassociation errors, far inside the
 // fast path's 1e-13 force bar, for two FMAs less per pair than a second step.
+#ifndef SL_RCP_NEWTON_STEPS
+#define SL_RCP_NEWTON_STEPS 1
+#endif
 __device__ __forceinline__ double rcp_f64(double x)
 {
-    const double y = __builtin_amdgcn_rcp(x);
-    const double e = __builtin_fma(-x, y, 1.0);
-    return __builtin_fma(y, e, y);
+    double y = __builtin_amdgcn_rcp(x);
+    for (int k = 0; k < SL_RCP_NEWTON_STEPS; k++) {
+        const double e = __builtin_fma(-x, y, 1.0);
+        y = __builtin_fma(y, e, y);
+    }
+    return y;
 }
 
 // 1/sqrt(x): hardware estimate and one Newton step y <- y (3/2 - x/2 y^2), about

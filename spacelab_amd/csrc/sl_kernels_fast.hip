@@ -96,6 +96,22 @@ __global__ void k_probe_rot(int* out)
 
 constexpr int GRAPH_MAX_STEPS = 128;   // steps per captured graph
 
+// A profiling tool loaded through rocprofiler-sdk (rocprofv3) is named in
+// ROCP_TOOL_LIBRARIES of the process it profiles.  Runs under rocprofv3 --kernel-trace
+// that replayed captured step graphs ended three times in host-side SIGSEGVs inside the
+// runtime, below hipGraphLaunch, while the same runs without the tool pass; graph
+// executables here are only destroyed on an idle stream and re-captured after every
+// reconfiguration.  With a tool attached the engine therefore launches every kernel
+// directly (the kernels and their order are the same).
+bool tool_attached()
+{
+    static const bool on = [] {
+        const char* v = std::getenv("ROCP_TOOL_LIBRARIES");
+        return v != nullptr && *v != 0;
+    }();
+    return on;
+}
+
 int err_hip(std::string* err, const char* what)
 {
     if (err) *err = std::string("fast path: ") + what;
@@ -565,10 +581,15 @@ int fast_configure(FastBuffers& fb, State& canon, const FastLayout& lay, const P
         return SL_E_ARG;
     }
     const bool same = fb.lay.n_sys == lay.n_sys && fb.lay.n == lay.n;
+    // captured graphs hold the buffer addresses and tile counts of the old layout: drop them
+    // (the stream is idle here) and the tuned split before any buffer is replaced
+    fast_clear_graphs(fb);
     fb.lay = lay;
     fb.cur = 0;
     fb.drifted = false;
     if (!same) {
+        fb.split = 0;
+        fb.n_tiles = 0;
         const int nblocks = lay.n_used / TB;
         fb.blk_sys_h.assign(std::max(nblocks, 1), 0);
         std::vector<int4> tiles;
@@ -687,7 +708,7 @@ int fast_steps(FastBuffers& fb, State& canon, int n_steps, long long* step_count
     if (rc) return rc;
     if (n_steps > 0) fb.canon_valid = false;
     const int ns = fb.lay.n_sys;
-    const bool graphs = fb.use_graphs && !fb.hook.before;
+    const bool graphs = fb.use_graphs && !fb.hook.before && !tool_attached();
     int nw = 0;
     int k = 0;
     auto record = [&](void) -> int {
