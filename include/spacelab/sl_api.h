@@ -340,6 +340,48 @@ int sl_mlp_train_step(sl_mlp* m, int batch, const float* x, const int* cls, cons
 const char* sl_mlp_last_error(const sl_mlp* m);
 void sl_mlp_destroy(sl_mlp* m);
 
+/* ---- allocation-free stepping ----------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory
+ * while they step: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  The same ordered step as sl_step (the same state and energies, bit
+ * for bit) on a state the caller keeps in device memory.  The step's scratch (energy
+ * partials, touching-pair counters, the non-finite flag) lives in a fixed buffer of the
+ * library, one per device, held until the process ends; no call allocates or frees.
+ * Every call on a device shares that device's buffer: calls that overlap on one device
+ * race.
+ *
+ * dev_state  SL_NFIELD * stride doubles on the current device, field f of ball i at
+ *            dev_state[f * stride + i] (stride >= n), fields in the order below.  The
+ *            caller sets position, velocity, spin, acc and aacc (as sl_init_forces or a
+ *            step leaves them), R, m and moi; the half-step and previous-position fields
+ *            are overwritten by the step.  Out: the state after the steps.
+ * first_step the step count before the call: step first_step + k (k = 1..n_steps) is a
+ *            write step when it is a multiple of write_every (> 0; <= 0: none).
+ * dev_energy device array of SL_STATIC_E_STRIDE doubles per write step, or NULL: PE, KE,
+ *            p.x, p.y, p.z, L.x, L.y, L.z, touching pairs of that step, and 1.0 if a
+ *            non-finite state appeared since the start of the call (else 0.0).
+ * stream     a hipStream_t, or NULL for the null stream.  The call only enqueues work.
+ * p          precision must be SL_PREC_FP64; p->order is not read.
+ * Returns the number of write steps, SL_E_CAPACITY when n > SL_STATIC_MAX_BALLS (the
+ * size the buffer holds), SL_E_ARG for other invalid arguments, SL_E_HIP on a runtime
+ * error. */
+#define SL_STATIC_MAX_BALLS 65536
+#define SL_STATIC_E_STRIDE 10
+#define SL_NFIELD 27
+#define SL_F_POS 0       /* x, y, z in fields 0, 1, 2; likewise for the vectors below */
+#define SL_F_VEL 3
+#define SL_F_W 6
+#define SL_F_ACC 9
+#define SL_F_AACC 12
+#define SL_F_VELH 15     /* half-step velocity (written by the step) */
+#define SL_F_WH 18       /* half-step spin (written by the step) */
+#define SL_F_POS_PREV 21 /* positions of the previous pair pass (written by the step) */
+#define SL_F_R 24
+#define SL_F_M 25
+#define SL_F_MOI 26
+int sl_step_ordered_static(const sl_params* p, int n, int stride, double* dev_state, long long first_step,
+                           int n_steps, int write_every, double* dev_energy, void* stream);
+
 /* ---- arithmetic probe ------------------------------------------------------ */
 /* q = a/b, s = sqrt(a), lg = log(a) element-wise on the device, compiled exactly as
  * the ordered kernels are.  Lets tests check that binary64 division and square
