@@ -444,6 +444,48 @@ def surrogate_eval():
     return out
 
 
+def ordered_shapes(sizes=(5000, 2200, 1000), steps=(60, 200, 200)):
+    """Every launch shape of the ordered pair pass on aggregates of the benchmark's kind
+    (config physics, dt = t_c/50): microseconds per step, the shape the tuner picks, and
+    whether every shape gave the same bits as shape 1 (state and last energies)."""
+    import spacelab_amd as sl
+    out = {}
+    for n, k in zip(sizes, steps):
+        g, p = workload(n)
+        q = p.replace(order=sl.ORDER_ORDERED)
+        row, ref = {}, None
+        with sl.Collider(8, q) as probe:
+            count = 1
+            while True:
+                try:
+                    probe.set_option("ordered_shape", count + 1)
+                except sl.SpacelabError:
+                    break
+                count += 1
+        for shape in list(range(1, count + 1)) + [0]:
+            with sl.Collider(n, q) as c:
+                c.set_option("ordered_shape", shape)
+                c.upload(g)
+                c.init_conditions()
+                c.step(5, 5)
+                c.synchronize()
+                t0 = time.perf_counter()
+                ew = c.step(k, k)
+                c.synchronize()
+                us = (time.perf_counter() - t0) / k * 1e6
+                st = c.download()
+                used = c.get_option("ordered_shape")
+            bits = np.concatenate([st.pos.ravel(), st.vel.ravel(), st.w.ravel(), st.acc.ravel(),
+                                   st.aacc.ravel(), ew[-1].as_array()]).view(np.uint64)
+            if ref is None:
+                ref = bits
+            key = "tuned" if shape == 0 else f"shape{shape}"
+            row[key] = {"us_per_step": us, "shape": used, "same_bits": bool(np.array_equal(bits, ref))}
+            log(f"ordered n={n} {key}: {us:.1f} us/step (shape {used})")
+        out[f"n{n}"] = row
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -454,7 +496,8 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=500)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["ensemble", "surrogate", "mlp", "collision", "e2e", "config0"],
+    ap.add_argument("--only", choices=["ensemble", "surrogate", "mlp", "collision", "e2e", "config0",
+                                       "ordered-shapes"],
                     default=None,
                     help="run only this part and print its object (for profiling)")
     ap.add_argument("--no-surrogate", action="store_true")
@@ -477,7 +520,7 @@ def main():
     import spacelab_amd as sl
     if a.only:
         out = {"ensemble": lambda: ensemble(steps=a.ensemble_steps), "surrogate": surrogate_eval,
-               "mlp": mlp_only, "collision": collision, "e2e": end_to_end,
+               "mlp": mlp_only, "collision": collision, "e2e": end_to_end, "ordered-shapes": ordered_shapes,
                "config0": lambda: config0(threads=min(16, os.cpu_count() or 1),
                                           single_core=not a.no_config0_single)}[a.only]()
         print(json.dumps({a.only: out}), flush=True)

@@ -138,6 +138,13 @@ void sl_destroy(sl_ctx* ctx);
 #define SL_OPT_GRAPHS 1
 #define SL_OPT_TILE_SPLIT 2
 #define SL_OPT_BALL_WAVES 3
+/*   SL_OPT_ORDERED_SHAPE launch shape of the SL_ORDER_ORDERED pair pass (balls per
+ *                      workgroup, evaluating wavefronts, partners per chunk): 0 (default)
+ *                      = chosen by timing each shape on the first step and stored with
+ *                      the tile splits, or 1..n to force one.  Every shape gives the same
+ *                      results bit for bit.  sl_get_option reports the shape in use (0
+ *                      before the first step). */
+#define SL_OPT_ORDERED_SHAPE 4
 int sl_set_option(sl_ctx* ctx, int option, int value);
 int sl_get_option(sl_ctx* ctx, int option, int* value);
 
@@ -387,6 +394,11 @@ int sl_step_ordered_static(const sl_params* p, int n, int stride, double* dev_st
  * the ordered kernels are.  Lets tests check that binary64 division and square
  * root are correctly rounded on the device, which bitwise parity relies on. */
 int sl_probe_divsqrt(int n, const double* a, const double* b, double* q, double* s, double* lg);
+
+/* q = a/b element-wise on the device as the ordered kernels divide by a distance or a
+ * mass: from the correctly rounded reciprocal y = 1/b, q = a y corrected twice by the
+ * FMA-exact residual a - b q.  Must equal the correctly rounded quotient bit for bit. */
+int sl_probe_div_rcp(int n, const double* a, const double* b, double* q);
 
 /* Lane layout of the wavefront register rotation the fast pair kernel relies on:
  * out[l] (binary32) and out[64+l] (binary64) must be (l+1)%64, out[128+l] (l+31)%64. */

@@ -29,7 +29,9 @@ LIB_PATH = os.path.join(_PKG, "lib", "libspacelab.so")
 PREC_FP64, PREC_MIXED = 0, 1
 ORDER_ORDERED, ORDER_FAST = 0, 1
 OPT_GRAPHS, OPT_TILE_SPLIT, OPT_BALL_WAVES = 1, 2, 3
-_OPTS = {"graphs": OPT_GRAPHS, "tile_split": OPT_TILE_SPLIT, "ball_waves": OPT_BALL_WAVES}
+OPT_ORDERED_SHAPE = 4
+_OPTS = {"graphs": OPT_GRAPHS, "tile_split": OPT_TILE_SPLIT, "ball_waves": OPT_BALL_WAVES,
+         "ordered_shape": OPT_ORDERED_SHAPE}
 
 ERRORS = {-1: "SL_E_ARG", -2: "SL_E_HIP", -3: "SL_E_STATE", -4: "SL_E_CAPACITY", -5: "SL_E_NOGPU"}
 
@@ -203,6 +205,7 @@ def lib():
     L.sl_mlp_destroy.argtypes = [vp]
     L.sl_mlp_destroy.restype = None
     L.sl_probe_divsqrt.argtypes = [C.c_int] + [dp] * 5
+    L.sl_probe_div_rcp.argtypes = [C.c_int] + [dp] * 3
     L.sl_probe_rotation.argtypes = [C.POINTER(C.c_int)]
     L.sl_probe_copy.argtypes = [C.c_ulonglong, C.c_int, C.POINTER(C.c_double)]
     _lib = L
@@ -520,7 +523,8 @@ class Collider:
 
     def set_option(self, name: str, value: int):
         """'graphs' (0/1), 'tile_split' (0 = autotune, 1, 2, 4) or 'ball_waves' (0 = by
-        size, 4, 8) of the fast engine."""
+        size, 4, 8) of the fast engine; 'ordered_shape' (0 = autotune, 1..n) of the
+        ordered pair pass."""
         self._check(lib().sl_set_option(self._ctx, _OPTS[name], int(value)), "sl_set_option")
 
     def get_option(self, name: str) -> int:
@@ -864,3 +868,15 @@ def probe_divsqrt(a: np.ndarray, b: np.ndarray):
     if rc:
         raise SpacelabError(rc, "sl_probe_divsqrt")
     return q, s, lg
+
+
+def probe_div_rcp(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Device a/b as the ordered kernels divide by a distance or a mass (through the
+    correctly rounded reciprocal and two residual corrections)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    q = np.empty_like(a)
+    rc = lib().sl_probe_div_rcp(len(a), _dp(a), _dp(b), _dp(q))
+    if rc:
+        raise SpacelabError(rc, "sl_probe_div_rcp")
+    return q

@@ -22,7 +22,7 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libspacelab.so")
 
 SOURCES = ["sl_api.hip", "sl_kernels_ordered.hip", "sl_kernels_fast.hip", "sl_util.hip",
-           "sl_batch.hip", "sl_mlp.hip", "sl_static.hip", "sl_host.cpp", "sl_collide.cpp", "sl_csv.cpp"]
+           "sl_batch.hip", "sl_mlp.hip", "sl_static.hip", "sl_host.cpp", "sl_tune.cpp", "sl_collide.cpp", "sl_csv.cpp"]
 HOSTDIR = os.path.join(PKG, "host")
 # host programs over the C ABI: (source, executable in lib/)
 PROGRAMS = [("collider_main.cpp", "spacelab_collider")]

@@ -27,9 +27,11 @@ struct sl_ctx {
     sl_params p{};
     double* base = nullptr;     // NFIELD * n_pad doubles
     State st{};
-    double* pe_part = nullptr;  // per-block PE partials
+    double* pe_part = nullptr;  // per-ball pair energies (write steps)
     double* e_part = nullptr;   // per-block KE/p/L partials
-    unsigned long long* cnt = nullptr;  // [0] contacts total, [1] contacts this write step
+    unsigned long long* cnt = nullptr;  // [0] contacts total, [1] contacts this write step, [2..3] tuner trials
+    int ord_shape = -1;         // launch shape of the ordered pair pass (-1: not chosen yet)
+    int forced_ord_shape = 0;   // 1 + shape forces it
     int* nonfinite = nullptr;
     double* h_pe = nullptr;     // pinned
     double* h_e = nullptr;      // pinned
@@ -230,14 +232,53 @@ int pair_pass(sl_ctx* c, bool write, bool kick, int* n_pe_blocks)
     if (write) HIPCHK(c, launch_zero_u64(c->cnt + 1, 1, c->stream));
     {
         ProfScope ps(c, 0);
-        HIPCHK(c, launch_pairs_ordered(c->st, P, c->n, write ? 1 : 0, c->pe_part, c->cnt, c->stream));
+        HIPCHK(c, launch_pairs_ordered(c->st, P, c->n, write ? 1 : 0, c->pe_part, c->cnt, c->stream,
+                                       c->ord_shape >= 0 ? c->ord_shape : ordered_default_shape(c->n)));
     }
     // one sphere: no pair pass ran and no PE partial was written
-    *n_pe_blocks = c->n > 1 ? ordered_blocks(c->n, nullptr) : 0;
+    *n_pe_blocks = c->n > 1 ? c->n : 0;   // per-ball pair energies
     {
         ProfScope ps(c, 1);
         HIPCHK(c, launch_kick(c->st, c->n, c->p.dt, write ? 1 : 0, kick ? 1 : 0, c->e_part, c->nonfinite, c->stream));
     }
+    return SL_OK;
+}
+
+// Launch shape of the ordered pair pass for this system: a forced one, a stored choice, or
+// each shape timed on the live (drifted) state -- one warm-up and three timed launches,
+// energies off, counters into the trial slots -- with the fastest stored.  Every shape
+// gives the same bits, so the choice changes only the time.  The trial launches write the
+// same accelerations the step's own pass then writes again.
+int tune_ordered(sl_ctx* c)
+{
+    if (c->forced_ord_shape > 0) {
+        c->ord_shape = c->forced_ord_shape - 1;
+        return SL_OK;
+    }
+    const std::string key = "ordered:n" + std::to_string(c->n) + ":grav" + std::to_string(c->p.far_gravity);
+    if (const int cached = tune_cache_get(key); cached >= 1 && cached <= ordered_shapes()) {
+        c->ord_shape = cached - 1;
+        return SL_OK;
+    }
+    const Phys P = phys_of(c->p);
+    const int ns = ordered_shapes();
+    std::vector<float> t(ns, 1e30f);
+    hipEvent_t e0 = next_event(c), e1 = next_event(c);
+    if (!e0 || !e1) return set_err(c, SL_E_HIP, "tuner events");
+    for (int s = 0; s < ns; s++) {
+        HIPCHK(c, launch_pairs_ordered(c->st, P, c->n, 0, c->pe_part, c->cnt + 2, c->stream, s));
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+        for (int r = 0; r < 3; r++) HIPCHK(c, launch_pairs_ordered(c->st, P, c->n, 0, c->pe_part, c->cnt + 2, c->stream, s));
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+        HIPCHK(c, hipEventSynchronize(e1));
+        HIPCHK(c, hipEventElapsedTime(&t[s], e0, e1));
+    }
+    c->ev_used -= 2;
+    int best = 0;
+    for (int s = 1; s < ns; s++)
+        if (t[s] < t[best]) best = s;
+    c->ord_shape = best;
+    tune_cache_put(key, best + 1, t.data(), ns);
     return SL_OK;
 }
 
@@ -314,12 +355,11 @@ int sl_create(sl_ctx** out, int n_max, const sl_params* p)
     };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(SL_E_HIP);
     if (hipMalloc(&c->base, sizeof(double) * (size_t)NFIELD * c->n_pad) != hipSuccess) return fail(SL_E_HIP);
-    // ordered_blocks(n) is largest for the smallest block size (4 balls), whatever n <= n_max
-    const int max_blocks = (n_max + 3) / 4 + 8;
+    const int max_blocks = n_max + 8;   // per-ball pair energies
     const int e_blocks = kick_blocks(n_max) + 8;
     if (hipMalloc(&c->pe_part, sizeof(double) * max_blocks) != hipSuccess) return fail(SL_E_HIP);
     if (hipMalloc(&c->e_part, sizeof(double) * 7 * e_blocks) != hipSuccess) return fail(SL_E_HIP);
-    if (hipMalloc(&c->cnt, sizeof(unsigned long long) * 2) != hipSuccess) return fail(SL_E_HIP);
+    if (hipMalloc(&c->cnt, sizeof(unsigned long long) * 4) != hipSuccess) return fail(SL_E_HIP);
     if (hipMalloc(&c->nonfinite, sizeof(int)) != hipSuccess) return fail(SL_E_HIP);
     if (hipHostMalloc(&c->h_pe, sizeof(double) * max_blocks, 0) != hipSuccess) return fail(SL_E_HIP);
     if (hipHostMalloc(&c->h_e, sizeof(double) * 7 * e_blocks, 0) != hipSuccess) return fail(SL_E_HIP);
@@ -328,7 +368,7 @@ int sl_create(sl_ctx** out, int n_max, const sl_params* p)
     if (fast_alloc(c->fb, c->n_pad, 1) != 0) return fail(SL_E_HIP);
     for (int f = 0; f < NFIELD; f++) c->st.f[f] = c->base + (size_t)f * c->n_pad;
     if (hipMemsetAsync(c->base, 0, sizeof(double) * (size_t)NFIELD * c->n_pad, c->stream) != hipSuccess ||
-        hipMemsetAsync(c->cnt, 0, sizeof(unsigned long long) * 2, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->cnt, 0, sizeof(unsigned long long) * 4, c->stream) != hipSuccess ||
         hipMemsetAsync(c->nonfinite, 0, sizeof(int), c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
         return fail(SL_E_HIP);
@@ -351,6 +391,7 @@ int sl_set_params(sl_ctx* c, const sl_params* p)
         rc = fast_set_phys(c->fb, c->st, &P, p->precision, c->stream, &c->err);
         if (rc) return rc;
     }
+    if (p->far_gravity != c->p.far_gravity) c->ord_shape = -1;
     c->p = *p;
     return SL_OK;
 }
@@ -376,6 +417,12 @@ int sl_set_option(sl_ctx* c, int option, int value)
         c->fb.forced_ball_waves = value;
         fast_clear_graphs(c->fb);
         return SL_OK;
+    case SL_OPT_ORDERED_SHAPE:
+        if (value < 0 || value > ordered_shapes())
+            return set_err(c, SL_E_ARG, "ordered shape must be 0 (tuned) or 1..%d (got %d)", ordered_shapes(), value);
+        c->forced_ord_shape = value;
+        c->ord_shape = -1;
+        return SL_OK;
     default:
         return set_err(c, SL_E_ARG, "unknown option %d", option);
     }
@@ -393,6 +440,9 @@ int sl_get_option(sl_ctx* c, int option, int* value)
         return SL_OK;
     case SL_OPT_BALL_WAVES:
         *value = fast_ball_waves(c->fb);
+        return SL_OK;
+    case SL_OPT_ORDERED_SHAPE:
+        *value = c->ord_shape + 1;
         return SL_OK;
     default:
         return set_err(c, SL_E_ARG, "unknown option %d", option);
@@ -444,6 +494,7 @@ int sl_upload(sl_ctx* c, int n, const double* pos3, const double* vel3, const do
     c->forces_valid = false;
     HIPCHK(c, hipMemcpyAsync(c->base, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (n != c->n) c->ord_shape = -1;
     c->n = n;
     c->step = 0;
     const int rc = configure_fast(c);
@@ -496,6 +547,10 @@ int sl_step(sl_ctx* c, int n_steps, int write_every, sl_energy* e_out, int e_cap
         c->step++;
         const bool write = write_every > 0 && (c->step % write_every) == 0;
         HIPCHK(c, launch_drift(c->st, c->n, c->p.dt, c->stream));
+        if (c->ord_shape < 0) {
+            const int rt = tune_ordered(c);
+            if (rt) return rt;
+        }
         int npe = 0;
         int rc = pair_pass(c, write, true, &npe);
         if (rc) return rc;

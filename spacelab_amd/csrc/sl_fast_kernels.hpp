@@ -108,6 +108,8 @@ struct TileArgs {
     FastAux aux;
     const Phys* sysP;       // per system
     const double4* bnd;     // per 64-ball block: bounding box of the centres (lo with max R, hi)
+    const float4* xl;       // mixed: per ball, position relative to its block's box centre and R, in units of L0
+    const float* gq_blk;    // mixed: per 64-ball block, the largest g/q of its balls (scaled weights)
     int n_pad;
     int n_tiles;
     const int4* tiles;      // {I, J, system, first block of the system}
@@ -211,16 +213,30 @@ struct JRecMix {
     float Rs, q, g, pad;
 };
 static_assert(sizeof(JRecMix) == sizeof(JRec64), "both records share the LDS staging area");
+// The mixed path's record for a far tile (every pair farther apart than S + h_min): all
+// binary32, ball j's block-local position shifted by the offset between the two blocks'
+// reference points, so a pair's displacement is one binary32 subtraction per axis.
+struct JRec32 {
+    float4 p;   // x, y, z in units of L0 relative to block I's reference point; R / L0
+    float4 w;   // cohesion and gravity weights (q, g), 0, 0
+};
+static_assert(sizeof(JRec32) <= sizeof(JRec64), "all records share the LDS staging area");
+
+// Reference point of a 64-ball block: the centre of its bounding box (lo, hi).
+__device__ __forceinline__ double3 block_ref(double4 lo, double4 hi)
+{
+    return make_double3(0.5 * (lo.x + hi.x), 0.5 * (lo.y + hi.y), 0.5 * (lo.z + hi.z));
+}
 
 // The pair sweep of every tile: the far-field force of each pair, with the per-pair
 // near test only in tiles the bounding-box test cannot rule out.  A near pair
 // (closer than S + h_min) contributes nothing here; it is recorded as a bit of the
 // partner masks of both balls, for k_ball.  SL_PREC_MIXED evaluates the factor in
-// binary32 from binary64 geometry, in units of L0 (median radius) and M0 (median
-// mass), as one sum for vdW and gravity: f = Kf (q_i q_j / (s t)^2 + g_i g_j / d^3)
-// with per-ball weights q = sqrt(Kv/Kf) (R/L0)^3, g = sqrt(Kg/Kf) m/M0, accumulated in
-// binary32 per tile (inside a dust aggregate the gravity term is some 1e-20 of the vdW
-// term, below the resolution of either format).  Each tile writes its i-side and
+// binary32, in units of L0 (median radius) and M0 (median mass), as one sum for vdW and
+// gravity: f = Kf (q_i q_j / (s t)^2 + g_i g_j / d^3) with per-ball weights
+// q = sqrt(Kv/Kf) (R/L0)^3, g = sqrt(Kg/Kf) m/M0, accumulated in binary32 per tile.  In
+// far tiles of non-write steps the geometry is binary32 too (block-local coordinates);
+// elsewhere it is binary64, rounded once the near test is done.  Each tile writes its i-side and
 // j-side sums and masks once, to slot J of the balls of block I and slot I of the
 // balls of block J, and its far-pair energy on write steps.
 template <int PREC, bool WRITE, bool GRAV, int SPLIT>
@@ -252,8 +268,22 @@ __global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
     const double xi = a.px[i], yi = a.py[i], zi = a.pz[i], Ri = a.R[i];
     const double qi = PREC == SL_PREC_MIXED ? (double)a.aux.q32[i] : a.aux.q[i];
     const double gi = PREC == SL_PREC_MIXED ? (double)a.aux.g32[i] : a.aux.g[i];
+    // mixed precision, far tile, no energies: the whole pair in binary32
+    const bool f32 = PREC == SL_PREC_MIXED && !WRITE && live && !test;
     if (live && h == 0) {
-        if constexpr (PREC == SL_PREC_MIXED) {
+        if (f32) {
+            const double3 ci = block_ref(a.bnd[2 * I], a.bnd[2 * I + 1]);
+            const double3 cj = block_ref(a.bnd[2 * J], a.bnd[2 * J + 1]);
+            const double L0inv = (double)a.L0inv;
+            const float4 xj = a.xl[j0];
+            JRec32 r;
+            r.p = make_float4(xj.x + (float)((cj.x - ci.x) * L0inv), xj.y + (float)((cj.y - ci.y) * L0inv),
+                              xj.z + (float)((cj.z - ci.z) * L0inv), xj.w);
+            r.w = make_float4(a.aux.q32[j0], a.aux.g32[j0], 0.f, 0.f);
+            JRec32* sm = reinterpret_cast<JRec32*>(sj[tl]);
+            sm[l] = r;
+            sm[l + TB] = r;
+        } else if constexpr (PREC == SL_PREC_MIXED) {
             JRecMix r;
             r.x = a.px[j0];
             r.y = a.py[j0];
@@ -291,7 +321,71 @@ __global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
         const double hmin2 = h_min * h_min, two_hmin = 2 * h_min;
         int s0, s1;
         sweep_range<SPLIT>(diag, h, s0, s1);
-        if constexpr (PREC == SL_PREC_MIXED) {
+        if (PREC == SL_PREC_MIXED && f32) {
+            // far tile: displacement, distance, the vdW and gravity factor and the sums
+            // all in binary32, in units of L0 and M0 (no near test: the box test cleared
+            // every pair)
+            const float4 xi4 = a.xl[i];
+            const float qif = a.aux.q32[i], gif = a.aux.g32[i];
+            const float r4i = 4.0f * xi4.w;
+            float fax = 0.f, fay = 0.f, faz = 0.f, fbx = 0.f, fby = 0.f, fbz = 0.f;
+            const JRec32* rec32 = reinterpret_cast<const JRec32*>(sj[tl]) + l;
+            // Gravity is left out of a tile whose every pair has it below 2^-27 of the vdW
+            // term: g_i g_j / d^3 over q_i q_j / (s t)^2 is at most (g/q)_I (g/q)_J dmax^5,
+            // because s, t <= d^2 <= dmax^2 (dmax: the largest distance between the blocks'
+            // boxes).  Added by the FMA it would be under half an ulp of the factor, so
+            // leaving it out changes no bit.
+            bool grav = GRAV;
+            if (GRAV) {
+                const double4 li = a.bnd[2 * I], ui = a.bnd[2 * I + 1], lj = a.bnd[2 * J], uj = a.bnd[2 * J + 1];
+                const double ex = fmax(uj.x - li.x, ui.x - lj.x), ey = fmax(uj.y - li.y, ui.y - lj.y);
+                const double ez = fmax(uj.z - li.z, ui.z - lj.z);
+                const double dmax = __builtin_sqrt(ex * ex + ey * ey + ez * ez) * (double)a.L0inv;
+                const double d2m = dmax * dmax;
+                const double bound = (double)a.gq_blk[I] * (double)a.gq_blk[J] * d2m * d2m * dmax;
+                grav = !(bound < 0x1p-27);
+            }
+            auto pair32 = [&](int s, auto gv) {
+                const float4 rp = rec32[s].p;
+                const float4 rw = rec32[s].w;
+                const float dx = rp.x - xi4.x, dy = rp.y - xi4.y, dz = rp.z - xi4.z;
+                const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                const float S = xi4.w + rp.w;
+                const float sv = __builtin_fmaf(-S, S, d2);
+                const float tv = __builtin_fmaf(r4i, rp.w, sv);
+                const float st = sv * tv;
+                float f = (qif * rw.x) * __builtin_amdgcn_rcpf(st * st);
+                if constexpr (decltype(gv)::value) {
+                    const float rq = __builtin_amdgcn_rsqf(d2);
+                    f = __builtin_fmaf(gif * rw.y, rq * rq * rq, f);
+                }
+                fax = __builtin_fmaf(f, dx, fax);
+                fay = __builtin_fmaf(f, dy, fay);
+                faz = __builtin_fmaf(f, dz, faz);
+                fbx = rot1(__builtin_fmaf(-f, dx, fbx));
+                fby = rot1(__builtin_fmaf(-f, dy, fby));
+                fbz = rot1(__builtin_fmaf(-f, dz, fbz));
+            };
+            {
+                constexpr int NS = TB / SPLIT;
+                if (GRAV && grav) {
+#pragma unroll SL_TILE_UNROLL
+                    for (int k = 0; k < NS; k++) pair32(s0 + k, std::true_type{});
+                } else {
+#pragma unroll SL_TILE_UNROLL
+                    for (int k = 0; k < NS; k++) pair32(s0 + k, std::false_type{});
+                }
+            }
+            const int back = (TB - (s1 & (TB - 1))) & (TB - 1);
+            if (back) {
+                fbx = rotk(fbx, back);
+                fby = rotk(fby, back);
+                fbz = rotk(fbz, back);
+            }
+            const double K = a.Kf;
+            ax = K * (double)fax; ay = K * (double)fay; az = K * (double)faz;
+            bx = K * (double)fbx; by = K * (double)fby; bz = K * (double)fbz;
+        } else if constexpr (PREC == SL_PREC_MIXED) {
             const float qif = (float)qi, gif = (float)gi, L0inv2 = a.L0inv2, L0inv = a.L0inv;
             const float r4i = 4.0f * (float)Ri * L0inv;
             float fax = 0.f, fay = 0.f, faz = 0.f, fbx = 0.f, fby = 0.f, fbz = 0.f;
@@ -560,10 +654,23 @@ struct BallArgs {
     const int* perm;    // engine slot -> the caller's ball index (fixes near-pair orientation)
     int kick, drift_next;
     double4* bnd;       // per 64-ball block: bounding box of the drifted positions (lo, hi)
+    float4* xl;         // per ball: drifted position relative to its block's box centre, R (units of L0)
+    double L0inv;
     double* e_blk;      // FAST_E_STRIDE doubles per 64-ball block (write steps)
     unsigned long long* blk_ctr;   // per 64-ball block: running sums of touching and near pairs
     int* flags;         // [1]: a non-finite state appeared
 };
+
+// Block-local binary32 position of ball l of a block whose bounding box is (lo, hi); padding
+// lanes get far, distinct, finite places (their weights are 0, so they add nothing).
+__device__ __forceinline__ float4 local_pos(double x, double y, double z, double R, bool valid, double4 lo,
+                                            double4 hi, double L0inv, int l)
+{
+    if (!valid) return make_float4(1e6f + 2048.f * (float)l, 1e6f, 1e6f, 0.f);
+    const double3 c = block_ref(lo, hi);
+    return make_float4((float)((x - c.x) * L0inv), (float)((y - c.y) * L0inv), (float)((z - c.z) * L0inv),
+                       (float)(R * L0inv));
+}
 
 constexpr int BPB = TB;              // balls per k_ball block
 constexpr int KB_C = 8;              // slot classes K = c, c + KB_C, ...: they fix the order of every sum
@@ -835,6 +942,7 @@ __global__ __launch_bounds__(kb_threads(NWV)) void k_ball(BallArgs a)
             a.bnd[2 * blockIdx.x] = lo;
             a.bnd[2 * blockIdx.x + 1] = hi;
         }
+        a.xl[i] = local_pos(pn.x, pn.y, pn.z, Rb, valid, lo, hi, a.L0inv, l);
     }
     if (WRITE) {
         // a 64-ball block lies inside one system, so its sums are per system
@@ -860,7 +968,7 @@ __global__ __launch_bounds__(kb_threads(NWV)) void k_ball(BallArgs a)
 // Bounding boxes of the 64-ball blocks (one wavefront per block), for positions that
 // did not come from k_ball's drift.
 __global__ __launch_bounds__(TB) void k_bounds(State s, const int* blk_sys, const int* sys_off, const int* sys_n,
-                                               double4* bnd)
+                                               double4* bnd, float4* xl, double L0inv)
 {
     const int l = threadIdx.x;
     const int b = blockIdx.x * TB + l;
@@ -868,11 +976,13 @@ __global__ __launch_bounds__(TB) void k_bounds(State s, const int* blk_sys, cons
     const bool valid = b - sys_off[sy] < sys_n[sy];
     const int bc = valid ? b : sys_off[sy];
     double4 lo, hi;
-    wave_aabb(s.f[PX][bc], s.f[PY][bc], s.f[PZ][bc], s.f[RAD][bc], valid, lo, hi);
+    const double x = s.f[PX][bc], y = s.f[PY][bc], z = s.f[PZ][bc], R = s.f[RAD][bc];
+    wave_aabb(x, y, z, R, valid, lo, hi);
     if (l == 0) {
         bnd[2 * blockIdx.x] = lo;
         bnd[2 * blockIdx.x + 1] = hi;
     }
+    xl[b] = local_pos(x, y, z, R, valid, lo, hi, L0inv, l);
 }
 
 // Start-of-call drift of the valid balls (each system with its own dt).

@@ -13,9 +13,13 @@ namespace sl {
 // ---- ordered (bitwise) path: sl_kernels_ordered.hip
 hipError_t launch_drift(const State& s, int n, double dt, hipStream_t st);
 hipError_t launch_copy_prev(const State& s, int n, hipStream_t st);
-int ordered_blocks(int n, int* bi);
-hipError_t launch_pairs_ordered(const State& s, const Phys& P, int n, int write, double* pe_part,
-                                unsigned long long* contacts, hipStream_t st);
+// Launch shapes of the pair pass (all give the same bits); shape = index below ordered_shapes().
+int ordered_shapes();
+int ordered_default_shape(int n);
+int ordered_blocks(int n, int shape);
+// pe_ball: on write steps, per ball i the energy of its pairs (i, j < i) in ascending j.
+hipError_t launch_pairs_ordered(const State& s, const Phys& P, int n, int write, double* pe_ball,
+                                unsigned long long* contacts, hipStream_t st, int shape);
 int kick_blocks(int n);
 hipError_t launch_kick(const State& s, int n, double dt, int write, int kick, double* e_part,
                        int* nonfinite, hipStream_t st);
@@ -75,6 +79,8 @@ struct FastBuffers {
     bool drifted = false;         // the next step's drift was already applied
     FastAux aux{};
     double4* bnd = nullptr;       // per 64-ball block: bounding box of the positions (lo, hi)
+    float4* xl = nullptr;         // per ball: position relative to its block's box centre, R (units of L0)
+    float* gq_blk = nullptr;      // per 64-ball block: largest g/q of the mixed path's scaled weights
     double L0 = 1, M0 = 1;        // length and mass units of the mixed path
     double Kf = 1;                // mixed path: force scale of the binary32 factor
     int* blk_sys = nullptr;       // device copies of the layout
@@ -138,5 +144,10 @@ void fast_clear_graphs(FastBuffers& fb);
 int fast_ball_waves(const FastBuffers& fb);
 hipError_t fast_reset_counters(FastBuffers& fb, hipStream_t st);
 int fast_read_counters(FastBuffers& fb, hipStream_t st, long long* contacts, long long* near_pairs);
+
+// ---- the autotuner's persistent cache: sl_tune.cpp
+std::string tune_key(int precision, int gravity, int n_sys, int n_used, int nb_max, int n_tiles);
+int tune_cache_get(const std::string& key);   // the stored split, 0 if none
+void tune_cache_put(const std::string& key, int split, const float* ms, int n);
 
 }  // namespace sl

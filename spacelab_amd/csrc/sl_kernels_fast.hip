@@ -67,6 +67,21 @@ __global__ void k_aux(State s, FastAux aux, int n, double sqrtC, double sqrtG, d
     aux.g32[b] = (float)(wg * (m * M0inv));
 }
 
+// Per 64-ball block (one wavefront each): the largest gravity-to-cohesion weight ratio
+// g/q of its balls, the mixed path's bound on where gravity can matter in a far tile.
+__global__ void k_gq_blk(FastAux aux, const int* blk_sys, const int* sys_off, const int* sys_n, float* gq)
+{
+    const int l = threadIdx.x;
+    const int b = blockIdx.x * TB + l;
+    const int sy = blk_sys[blockIdx.x];
+    const bool valid = b - sys_off[sy] < sys_n[sy];
+    const float q = aux.q32[b], g = aux.g32[b];
+    // a ball without cohesion (q = 0) but with gravity has no bound: +inf keeps gravity on
+    float r = !valid ? 0.f : (q > 0.f ? g / q : (g > 0.f ? __builtin_huge_valf() : 0.f));
+    for (int m = 1; m < TB; m <<= 1) r = fmaxf(r, __shfl_xor(r, m, TB));
+    if (l == 0) gq[blockIdx.x] = r;
+}
+
 // dst[k] = src[perm[k]] (to engine order) for fields [f0, f1)
 __global__ void k_gather(State dst, State src, const int* perm, int n, int f0, int f1)
 {
@@ -160,6 +175,8 @@ TileArgs tile_args(FastBuffers& fb)
     a.pz = c.f[PZ];
     a.R = c.f[RAD];
     a.aux = fb.aux;
+    a.xl = fb.xl;
+    a.gq_blk = fb.gq_blk;
     a.sysP = fb.sysP;
     a.bnd = fb.bnd;
     a.n_pad = fb.n_pad;
@@ -183,13 +200,16 @@ bool split_ready(const FastBuffers& fb) { return fb.split != 0 && fb.split_key =
 
 // Autotuner: the number of wavefronts sharing one far tile trades per-wave set-up
 // against occupancy and load balance, which depend on the systems' sizes and the
-// machine.  Each choice is warmed up once and then timed over three rounds of five
-// launches on the live state (HIP events on the engine's stream); the choice with the
-// fastest round is kept.  The trial launches only write partial sums that the real
-// launch overwrites.
+// machine.  A choice stored for this configuration (sl_tune.cpp) is used as it is;
+// otherwise each choice is warmed up once and then timed over three rounds of five
+// launches on the live state (HIP events on the engine's stream), and the one kept is
+// stored.  The trial launches only write partial sums that the real launch overwrites.
 int tune_split(FastBuffers& fb, const TileArgs& a, hipStream_t st)
 {
     if (fb.forced_split == 1 || fb.forced_split == 2 || fb.forced_split == 4) return fb.forced_split;
+    const std::string key = tune_key(fb.precision, gravity_on(fb) ? 1 : 0, fb.lay.n_sys, fb.lay.n_used,
+                                     fb.lay.nb_max, fb.n_tiles);
+    if (const int cached = tune_cache_get(key)) return cached;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return 1;
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -217,9 +237,14 @@ int tune_split(FastBuffers& fb, const TileArgs& a, hipStream_t st)
     // the smallest split within 2% of the fastest: splits that time alike then give the
     // same choice (and the same sums) from run to run more often
     const float fastest = std::min(t_min[0], std::min(t_min[1], t_min[2]));
+    int pick = 1;
     for (int k = 0; k < 3; k++)
-        if (t_min[k] <= 1.02f * fastest) return choices[k];
-    return 1;
+        if (t_min[k] <= 1.02f * fastest) {
+            pick = choices[k];
+            break;
+        }
+    tune_cache_put(key, pick, t_min, 3);
+    return pick;
 }
 
 int launch_tiles_step(FastBuffers& fb, bool write, hipStream_t st, std::string* err)
@@ -255,6 +280,8 @@ int launch_ball_step(FastBuffers& fb, bool write, bool kick, bool drift_next, hi
     a.kick = kick;
     a.drift_next = drift_next;
     a.bnd = fb.bnd;
+    a.xl = fb.xl;
+    a.L0inv = 1.0 / fb.L0;
     a.e_blk = fb.e_blk;
     a.blk_ctr = fb.blk_ctr;
     a.flags = fb.flags;
@@ -279,7 +306,8 @@ int launch_ball_step(FastBuffers& fb, bool write, bool kick, bool drift_next, hi
 
 int launch_bounds(FastBuffers& fb, hipStream_t st, std::string* err)
 {
-    k_bounds<<<fb.lay.n_used / TB, TB, 0, st>>>(fb.eng[fb.cur], fb.blk_sys, fb.sys_off, fb.sys_n, fb.bnd);
+    k_bounds<<<fb.lay.n_used / TB, TB, 0, st>>>(fb.eng[fb.cur], fb.blk_sys, fb.sys_off, fb.sys_n, fb.bnd, fb.xl,
+                                                1.0 / fb.L0);
     if (hipGetLastError() != hipSuccess) return err_hip(err, "block bounds");
     return 0;
 }
@@ -495,6 +523,8 @@ int fast_alloc(FastBuffers& fb, int n_pad, int max_sys)
     if (hipMalloc(&fb.aux.q32, sizeof(float) * n_pad) != hipSuccess) return -1;
     if (hipMalloc(&fb.aux.g32, sizeof(float) * n_pad) != hipSuccess) return -1;
     if (hipMalloc(&fb.bnd, sizeof(double4) * 2 * nblocks) != hipSuccess) return -1;
+    if (hipMalloc(&fb.xl, sizeof(float4) * n_pad) != hipSuccess) return -1;
+    if (hipMalloc(&fb.gq_blk, sizeof(float) * nblocks) != hipSuccess) return -1;
     if (hipMalloc(&fb.blk_sys, sizeof(int) * nblocks) != hipSuccess) return -1;
     if (hipMalloc(&fb.sys_off, sizeof(int) * max_sys) != hipSuccess) return -1;
     if (hipMalloc(&fb.sys_n, sizeof(int) * max_sys) != hipSuccess) return -1;
@@ -524,7 +554,7 @@ void fast_clear_graphs(FastBuffers& fb)
 void fast_free(FastBuffers& fb)
 {
     fast_clear_graphs(fb);
-    void* ptrs[] = {fb.eng_base, fb.perm, fb.aux.q, fb.aux.g, fb.aux.q32, fb.aux.g32, fb.bnd, fb.blk_sys,
+    void* ptrs[] = {fb.eng_base, fb.perm, fb.aux.q, fb.aux.g, fb.aux.q32, fb.aux.g32, fb.bnd, fb.xl, fb.gq_blk, fb.blk_sys,
                     fb.sys_off, fb.sys_n, fb.sys_nb, fb.sysP, fb.tiles, fb.part, fb.nbm, fb.flags,
                     fb.blk_ctr, fb.e_blk, fb.pe_tile};
     for (void* p : ptrs)
@@ -568,6 +598,8 @@ int fast_set_phys(FastBuffers& fb, State& canon, const Phys* P, int precision, h
     for (int k = 0; k < 2; k++)
         k_pad_sys<<<(fb.n_pad + 255) / 256, 256, 0, st>>>(fb.eng[k], fb.aux, fb.blk_sys, fb.sys_off, fb.sys_n,
                                                           L.n_used, fb.n_pad);
+    if (L.n_used > 0)
+        k_gq_blk<<<L.n_used / TB, TB, 0, st>>>(fb.aux, fb.blk_sys, fb.sys_off, fb.sys_n, fb.gq_blk);
     if (hipGetLastError() != hipSuccess) return err_hip(err, "per-ball constants");
     fast_clear_graphs(fb);
     return 0;

@@ -5,22 +5,19 @@
 // ColliderSingleCore.cpp:100-103).  Floating-point addition is not associative,
 // so the GPU reproduces that order with one sequential accumulator per (ball,
 // component), while the expensive part -- evaluating the pair -- runs on every
-// lane of the block:
+// producer lane of the workgroup (k_pairs_ordered below: BI balls per workgroup,
+// partners in chunks, evaluation and summation of consecutive chunks overlapped).
+// Far pairs contribute +0.0 to aacc and j == i contributes +0.0 to both; an
+// accumulator that starts at +0.0 under round-to-nearest never becomes -0.0,
+// so these additions leave it bit-for-bit unchanged and are skipped.
 //
-//   block = 256 threads, BI balls i, j swept in chunks of CJ (64 for BI = 16, else 128):
-//     produce: thread (il, jq) evaluates pairs (i0+il, j0+jq+JQ*k) in the
-//              reference orientation (A = max(i,j), B = min(i,j)) and writes
-//              the signed contribution for ball i into LDS;
-//     consume: 6*BI threads, one per (ball, component), add the CJ chunk
-//              contributions in ascending j into their running sums.
-//   Far pairs contribute +0.0 to aacc and j == i contributes +0.0 to both; an
-//   accumulator that starts at +0.0 under round-to-nearest never becomes -0.0,
-//   so these additions leave it bit-for-bit unchanged.
-//
-// PE is summed per thread and per block in a fixed order (deterministic, but not
-// the reference's single running sum, so it agrees to rounding only).
+// PE is summed per ball A over its partners B < A in ascending B, and the per-ball sums
+// over A in ascending order by the caller: the same for every launch shape, but grouped
+// per ball rather than the reference's single running sum, so it agrees to rounding only.
 #include "sl_common.hpp"
 #include "sl_pair_ref.hpp"
+
+#include <type_traits>
 
 #pragma clang fp contract(off)
 
@@ -98,106 +95,295 @@ __global__ void k_copy_prev(State s, int n)
     s.f[QZ][b] = s.f[PZ][b];
 }
 
-// Limited to 128 VGPRs (four wavefronts per SIMD; a 20-byte spill) for the 4- and 8-ball
-// blocks: 8% faster at 5000 spheres, 6.5% at 10,000 (the 16-ball block is held to three by
-// its LDS anyway).
-template <int BI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_pairs_ordered(State s, Phys P, int n, int write,
-                                                       double* __restrict__ pe_part,
-                                                       unsigned long long* __restrict__ contacts)
+// One touching pair (i, j) in the reference's contact arm (ColliderSingleCore.cpp:119-204),
+// in its orientation A = max(i, j): ball i's acc contribution to c3[0], c3[cs], c3[2 cs], its
+// aacc contribution likewise to a3, and the pair energy (write steps, i = A) to *pe.  The
+// state is one array of fields, field f at base + f stride (how every caller lays it out),
+// passed as its base so that the caller has no field table to spill for the call.
+__device__ __forceinline__ void contact_pair_body(const double* base, long stride, const Phys& P, int write, int i,
+                                                  int j, double ymi, double* c3, double* a3, int cs, double* pe)
 {
-    constexpr int NT = 256;
-    constexpr int CJ = BI == 16 ? 64 : 128;   // partners per chunk (one barrier pair each)
-    constexpr int JQ = NT / BI;            // threads sharing one ball i
-    constexpr int CS = CJ * BI + 8;         // component stride, padded against bank conflicts
-    __shared__ double cont[6 * CS];
-    __shared__ double red[NT];
+    State s;
+    for (int f = 0; f < NFIELD; f++) s.f[f] = const_cast<double*>(base + f * stride);
+    const bool iA = i > j;
+    const D3 pi = d3(s.f[PX][i], s.f[PY][i], s.f[PZ][i]);
+    const D3 pj = d3(s.f[PX][j], s.f[PY][j], s.f[PZ][j]);
+    const double Ri = s.f[RAD][i], mi = s.f[MASS][i], Ii = s.f[MOI][i];
+    const double Rj = s.f[RAD][j], mj = s.f[MASS][j];
+    PairIn in;
+    in.pA = iA ? pi : pj;
+    in.pB = iA ? pj : pi;
+    in.RA = iA ? Ri : Rj;
+    in.RB = iA ? Rj : Ri;
+    in.mA = iA ? mi : mj;
+    in.mB = iA ? mj : mi;
+    PairOut o;
+    GlobalContact gcx{s, iA ? i : j, iA ? j : i};
+    pair_ref(P, write && iA, in, gcx, o);
+    // acc[A] += F/m[A] or acc[B] -= F/m[B]; aacc[A] += tA/moi[A], aacc[B] += tB/moi[B] (:187-188, 249-250)
+    D3 a = d3(div_rcp(o.F.x, mi, ymi), div_rcp(o.F.y, mi, ymi), div_rcp(o.F.z, mi, ymi));
+    if (!iA) a = -a;
+    const D3 aa = (iA ? o.tA : o.tB) / Ii;
+    c3[0] = a.x;
+    c3[cs] = a.y;
+    c3[2 * cs] = a.z;
+    a3[0] = aa.x;
+    a3[cs] = aa.y;
+    a3[2 * cs] = aa.z;
+    *pe = iA ? o.pe : 0.0;
+}
+
+// A partner's position, radius and mass as the far-field arm reads them (48 bytes: three
+// 16-byte LDS reads).
+struct PartnerRec {
+    double x, y, z, R, m, pad;
+};
+
+// One workgroup owns BI balls and sweeps every partner j in chunks of CJ.  NPW wavefronts
+// evaluate pairs (producers); the running sums are kept by consumer lanes, one per (ball,
+// quantity): acc x, y, z, aacc x, y, z and, on write steps, the pair energy of the ball as
+// A.  With DED the consumer lanes are a wavefront of their own; otherwise they are the first
+// 7 BI lanes of wavefront 0, which then also produces.
+//   produce (chunk k): thread (il, jq) evaluates the pairs (i0 + il, j0 + jq + JQ q) in the
+//     reference orientation and writes ball i's signed acc contribution (for a touching
+//     pair also its aacc contribution and a bit of the chunk's contact mask; on write
+//     steps the pair energy when i is A) to LDS buffer k mod 2;
+//   consume (chunk k - 1, at the same time): each consumer lane adds its ball's CJ
+//     contributions of its quantity in ascending j (aacc: only at the masked positions).
+// One barrier per chunk separates the two buffers' roles.  Every sum has the same order
+// whatever BI, NPW, CJ and DED are, so all launch shapes give the same bits (the energies
+// included: per-ball sums, added over the balls in ascending order by the caller).
+// The far arm divides by dist and by m_i through correctly rounded reciprocals
+// (div_rcp): the same quotients as the reference's divisions, for five FMA-pipe
+// instructions each instead of a full division.
+__device__ __attribute__((noinline)) void contact_pair(const double* base, long stride, const Phys& P, int write,
+                                                       int i, int j, double ymi, double* c3, double* a3, int cs,
+                                                       double* pe)
+{
+    contact_pair_body(base, stride, P, write, i, j, ymi, c3, a3, cs, pe);
+}
+
+// CALL: the contact arm out of line (its registers weigh only on the calls) or inline.
+// WPE: the wavefronts per SIMD the register budget is held to (0: the compiler's choice).
+template <int BI, int NPW, int CJ, bool DED, bool CALL, int WPE>
+__global__ __launch_bounds__((NPW + (DED ? 1 : 0)) * 64) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+void k_pairs_ordered(State s, Phys P, int n, int write, double* __restrict__ pe_ball,
+                     unsigned long long* __restrict__ contacts)
+{
+    constexpr int NP = NPW * 64;            // producer threads
+    constexpr int JQ = NP / BI;             // producer threads per ball
+    constexpr int KP = CJ / JQ;             // pairs per producer thread and chunk
+    constexpr int MW = (CJ + 63) / 64;      // contact-mask words per ball and chunk
+    constexpr int CS = CJ * BI + 4;         // quantity stride (staggers the consumer's banks)
+    constexpr int NQ = 7;                   // acc xyz, aacc xyz, pair energy
+    static_assert(NP % BI == 0 && CJ % JQ == 0 && KP >= 1, "chunk shape");
+    static_assert(NQ * BI <= 64, "one consumer lane per (ball, quantity)");
+    constexpr int NT = (NPW + (DED ? 1 : 0)) * 64;   // threads of the workgroup
+    constexpr int NL = (CJ + NT - 1) / NT;           // partner records each thread stages per chunk
+    __shared__ double cb[2][NQ * CS];
+    __shared__ unsigned long long cmask[2][BI][MW];
+    __shared__ PartnerRec sp[2][CJ];                 // the partners of chunks k and k + 1
+    __shared__ unsigned ccount;
 
     const int t = threadIdx.x;
-    const int il = t % BI;
-    const int jq = t / BI;
-    const int i = blockIdx.x * BI + il;
-    const bool iv = i < n;
-    const int ic = iv ? i : 0;
-    const D3 pi = d3(s.f[PX][ic], s.f[PY][ic], s.f[PZ][ic]);
-    const double Ri = s.f[RAD][ic], mi = s.f[MASS][ic], Ii = s.f[MOI][ic];
+    const int w = __builtin_amdgcn_readfirstlane(t / 64);
+    if (t < 2 * BI * MW) (&cmask[0][0][0])[t] = 0ull;
+    if (t == 0) ccount = 0;
 
-    // consumer role: (ball il, component cc) for t < 6*BI
-    const int cc = t / BI;
-    double run = 0.0;
-    double pe = 0.0;
+    // producer role
+    const int il = t % BI;
+    const int jq = (t / BI) % JQ;
+    const int i = blockIdx.x * BI + il;
+    const bool iv = w < NPW && i < n;
+    const int ic = i < n ? i : 0;
+    const D3 pi = d3(s.f[PX][ic], s.f[PY][ic], s.f[PZ][ic]);
+    const double Ri = s.f[RAD][ic], mi = s.f[MASS][ic];
+    const double ymi = 1.0 / mi;             // divisions by m_i through div_rcp
+    // ball i's factors of the reference's products when it is A (2 Ra, 4 Ra, Ha/6*64 Ra^3, G mA)
+    const double two_Ri = 2 * Ri, four_Ri = 4 * Ri;
+    const double Hc = P.Ha / 6 * 64;
+    const double prefA = Hc * Ri * Ri * Ri;
+    const double Gmi = P.G * mi;
     unsigned cnt = 0;
 
-    for (int j0 = 0; j0 < n; j0 += CJ) {
-#pragma unroll 1
-        for (int k = 0; k < CJ / JQ; k++) {
-            const int jl = jq + JQ * k;
-            const int j = j0 + jl;
-            double c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
-            if (iv && j < n && j != i) {
-                const bool iA = i > j;
-                const int A = iA ? i : j, B = iA ? j : i;
-                const D3 pj = d3(s.f[PX][j], s.f[PY][j], s.f[PZ][j]);
-                const double Rj = s.f[RAD][j], mj = s.f[MASS][j];
-                PairIn in;
-                in.pA = iA ? pi : pj;
-                in.pB = iA ? pj : pi;
-                in.RA = iA ? Ri : Rj;
-                in.RB = iA ? Rj : Ri;
-                in.mA = iA ? mi : mj;
-                in.mB = iA ? mj : mi;
-                PairOut o;
-                GlobalContact gc{s, A, B};
-                pair_ref(P, write && iA, in, gc, o);
-                if (iA) {
-                    // acc[A] += F/m[A]; aacc[A] += tA/moi[A]   (:187, :249)
-                    const D3 a = o.F / mi;
-                    c0 = a.x; c1 = a.y; c2 = a.z;
-                    if (o.contact) {
-                        const D3 aa = o.tA / Ii;
-                        c3 = aa.x; c4 = aa.y; c5 = aa.z;
+    // consumer role: lane c of the consumer wavefront, (ball c % BI, quantity c / BI)
+    constexpr int CW = DED ? NPW : 0;       // the consumer wavefront
+    const int c = t - CW * 64;
+    const int cl = c % BI, cq = c / BI;
+    const bool consumer = w == CW && c < NQ * BI;
+    double run = 0.0;
+
+    // Partner records are staged in LDS a chunk ahead: the loads of chunk k + 1 are issued
+    // before chunk k is evaluated and stored after it, so their latency is hidden.
+    const int nch = (n + CJ - 1) / CJ;
+    PartnerRec nx[NL];
+    auto load_chunk = [&](int kc) {
+#pragma unroll
+        for (int u = 0; u < NL; u++) {
+            const int jl = t + u * NT;
+            const int j = kc * CJ + jl;
+            if (jl < CJ && j < n)
+                nx[u] = PartnerRec{s.f[PX][j], s.f[PY][j], s.f[PZ][j], s.f[RAD][j], s.f[MASS][j], 0.0};
+        }
+    };
+    auto store_chunk = [&](int kc) {
+#pragma unroll
+        for (int u = 0; u < NL; u++) {
+            const int jl = t + u * NT;
+            if (jl < CJ && kc * CJ + jl < n) sp[kc & 1][jl] = nx[u];
+        }
+    };
+    load_chunk(0);
+    store_chunk(0);
+    __syncthreads();
+    for (int k = 0; k <= nch; k++) {
+        const int b = k & 1;
+        if (k + 1 < nch) load_chunk(k + 1);
+        if (w == CW && k > 0) {
+            // sums of chunk k - 1
+            const int bb = b ^ 1;
+            const int m = (n - (k - 1) * CJ) < CJ ? (n - (k - 1) * CJ) : CJ;
+            if (consumer) {
+                const double* src = &cb[bb][cq * CS + cl];
+                if (cq < 3 || (cq == 6 && write)) {
+                    int jl = 0;
+                    for (; jl + 8 <= m; jl += 8) {
+                        double v[8];
+#pragma unroll
+                        for (int u = 0; u < 8; u++) v[u] = src[(jl + u) * BI];
+#pragma unroll
+                        for (int u = 0; u < 8; u++) run = run + v[u];
                     }
-                    pe = pe + o.pe;
-                    cnt += o.contact;
-                } else {
-                    // acc[B] -= F/m[B]; aacc[B] += tB/moi[B]   (:188, :250)
-                    const D3 a = o.F / mi;
-                    c0 = -a.x; c1 = -a.y; c2 = -a.z;
-                    if (o.contact) {
-                        const D3 aa = o.tB / Ii;
-                        c3 = aa.x; c4 = aa.y; c5 = aa.z;
+                    for (; jl < m; jl++) run = run + src[jl * BI];
+                } else if (cq < 6) {
+                    // aacc: only touching pairs contribute (the others add +0.0, a no-op on
+                    // a sum that starts at +0.0), in ascending j
+                    for (int wd = 0; wd < MW; wd++) {
+                        unsigned long long mk = cmask[bb][cl][wd];
+                        while (mk) {
+                            const int bit = __ffsll((long long)mk) - 1;
+                            mk &= mk - 1;
+                            run = run + src[(wd * 64 + bit) * BI];
+                        }
                     }
                 }
             }
-            const int o_ = jl * BI + il;
-            cont[0 * CS + o_] = c0;
-            cont[1 * CS + o_] = c1;
-            cont[2 * CS + o_] = c2;
-            cont[3 * CS + o_] = c3;
-            cont[4 * CS + o_] = c4;
-            cont[5 * CS + o_] = c5;
+            // every lane of this wavefront has read the masks: clear them for chunk k + 1
+            __builtin_amdgcn_wave_barrier();
+            if (c >= 0 && c < BI * MW) (&cmask[bb][0][0])[c] = 0ull;
         }
-        __syncthreads();
-        if (t < 6 * BI) {
-            const double* src = cont + cc * CS + il;
-            const int m = (n - j0) < CJ ? (n - j0) : CJ;
-            for (int jl = 0; jl < m; jl++) run = run + src[jl * BI];
+        if (w < NPW && k < nch) {
+            const int j0 = k * CJ;
+            const int i0 = blockIdx.x * BI;
+            unsigned pend = 0;   // pairs q of this chunk that touch: evaluated after the sweep
+            // One far-field pair (i, j) in the reference's orientation A = max(i, j): ball i
+            // is A when IA (ColliderSingleCore.cpp:104-110, 211-227).  Writes ball i's acc
+            // contribution (and its energy on write steps), or marks the pair touching.
+            auto far_pair = [&](auto ia, int q) {
+                constexpr bool IA = decltype(ia)::value;
+                const int jl = jq + JQ * q;
+                const int j = j0 + jl;
+                double c0 = 0, c1 = 0, c2 = 0, u = 0;
+                if (iv && j < n && j != i) {
+                    const PartnerRec& pr = sp[b][jl];
+                    const D3 pj = d3(pr.x, pr.y, pr.z);
+                    const double Rj = pr.R, mj = pr.m;
+                    const double S = IA ? Ri + Rj : Rj + Ri;
+                    const D3 r = IA ? pj - pi : pi - pj;               // x_B - x_A
+                    const double dist = norm(r);
+                    const double overlap = S - dist;
+                    if (overlap > 0) {
+                        pend |= 1u << q;
+                    } else {
+                        double h = ::fabs(overlap);
+                        if (h < P.h_min) h = P.h_min;
+                        const double h2 = h * h;
+                        // vdw_factor with Ra = R_A, Rb = R_B: 2 Ra h, 2 Rb h, 4 Ra Rb and
+                        // Ha/6*64 Ra Ra Ra Rb Rb Rb in the reference's product order
+                        const double twoRah = IA ? two_Ri * h : (2 * Rj) * h;
+                        const double twoRbh = IA ? (2 * Rj) * h : two_Ri * h;
+                        const double sv = h2 + twoRah + twoRbh;
+                        const double tv = sv + (IA ? four_Ri * Rj : (4 * Rj) * Ri);
+                        const double quot = (IA ? h + Ri + Rj : h + Rj + Ri) / (sv * sv * tv * tv);
+                        const double fv = IA ? prefA * Rj * Rj * Rj * quot : Hc * Rj * Rj * Rj * Ri * Ri * Ri * quot;
+                        const double yd = 1.0 / dist;
+                        const D3 rh = d3(div_rcp(r.x, dist, yd), div_rcp(r.y, dist, yd), div_rcp(r.z, dist, yd));
+                        D3 F = rh * fv;
+                        if (P.far_gravity) F = F + rh * ((IA ? Gmi * mj : P.G * mj * mi) / (dist * dist));
+                        // acc[A] += F/m[A]  or  acc[B] -= F/m[B]   (:187-188, 249-250)
+                        c0 = div_rcp(F.x, mi, ymi);
+                        c1 = div_rcp(F.y, mi, ymi);
+                        c2 = div_rcp(F.z, mi, ymi);
+                        if (!IA) {
+                            c0 = -c0;
+                            c1 = -c1;
+                            c2 = -c2;
+                        } else if (write) {                            // :228-240
+                            u = vdw_energy(P.Ha, Ri, Rj, S, h);
+                            if (P.far_gravity) u = u + -P.G * mi * mj / dist;
+                        }
+                    }
+                }
+                const int o_ = jl * BI + il;
+                cb[b][0 * CS + o_] = c0;
+                cb[b][1 * CS + o_] = c1;
+                cb[b][2 * CS + o_] = c2;
+                if (write) cb[b][6 * CS + o_] = u;
+            };
+            if (j0 + CJ <= i0) {
+                // every partner of the chunk precedes every ball of the block: i is A
+#pragma unroll 1
+                for (int q = 0; q < KP; q++) far_pair(std::true_type{}, q);
+            } else if (j0 >= i0 + BI) {
+                // every partner follows: i is B
+#pragma unroll 1
+                for (int q = 0; q < KP; q++) far_pair(std::false_type{}, q);
+            } else {
+#pragma unroll 1
+                for (int q = 0; q < KP; q++) {
+                    if (i > j0 + jq + JQ * q)
+                        far_pair(std::true_type{}, q);
+                    else
+                        far_pair(std::false_type{}, q);
+                }
+            }
+            // touching pairs: the reference's contact arm in full (:119-204), out of line so
+            // that its registers do not weigh on the far-field sweep
+            while (pend) {
+                const int q = __ffs(pend) - 1;
+                pend &= pend - 1;
+                const int jl = jq + JQ * q;
+                const int o_ = jl * BI + il;
+                double pc = 0;
+                if constexpr (CALL)
+                    contact_pair(s.f[0], s.f[1] - s.f[0], P, write, i, j0 + jl, ymi, &cb[b][o_], &cb[b][3 * CS + o_],
+                                 CS, &pc);
+                else
+                    contact_pair_body(s.f[0], s.f[1] - s.f[0], P, write, i, j0 + jl, ymi, &cb[b][o_],
+                                      &cb[b][3 * CS + o_], CS, &pc);
+                if (write) cb[b][6 * CS + o_] = pc;
+                atomicOr(&cmask[b][il][jl >> 6], 1ull << (jl & 63));
+                if (i > j0 + jl) cnt++;
+            }
         }
+        // chunk k + 1's partners into the buffer chunk k - 1 used (all reads of it ended
+        // before the previous barrier)
+        if (k + 1 < nch) store_chunk(k + 1);
         __syncthreads();
     }
 
-    if (t < 6 * BI && iv) {
-        const int fld = cc < 3 ? (AX + cc) : (BX + cc - 3);
-        s.f[fld][i] = run;
-    }
-    if (write) {
-        const double tot = block_sum<NT>(pe, red);
-        if (t == 0) pe_part[blockIdx.x] = tot;
+    if (consumer) {
+        const int ib = blockIdx.x * BI + cl;
+        if (ib < n) {
+            if (cq < 3)
+                s.f[AX + cq][ib] = run;
+            else if (cq < 6)
+                s.f[BX + cq - 3][ib] = run;
+            else if (write)
+                pe_ball[ib] = run;
+        }
     }
     // contact count (integer, order-independent)
-    __shared__ unsigned ccount;
-    if (t == 0) ccount = 0;
-    __syncthreads();
     if (cnt) atomicAdd(&ccount, cnt);
     __syncthreads();
     // contacts[0]: the running total; contacts[1]: this write step's touching pairs
@@ -273,28 +459,52 @@ hipError_t launch_copy_prev(const State& s, int n, hipStream_t st)
     return hipGetLastError();
 }
 
-int ordered_blocks(int n, int* bi)
+// Launch shapes of the ordered pair pass: {balls per workgroup, producer wavefronts, partners
+// per chunk, consumer wavefront of its own}.  All give the same results bit for bit.
+namespace {
+struct OrdShape {
+    int bi, npw, cj, ded;
+};
+constexpr OrdShape ORD_SHAPES[] = {
+    {4, 4, 128, 0}, {4, 2, 64, 0}, {4, 4, 64, 0}, {8, 4, 32, 0}, {2, 2, 128, 0},
+    {8, 4, 64, 0},  {2, 2, 128, 0}, {4, 4, 64, 0}, {8, 4, 32, 0},
+};
+constexpr int N_ORD_SHAPES = (int)(sizeof(ORD_SHAPES) / sizeof(ORD_SHAPES[0]));
+}  // namespace
+
+int ordered_shapes() { return N_ORD_SHAPES; }
+
+// The shape used when none is chosen (the allocation-free entry point, and until the
+// tuner has run), from timings on MI355X: up to about 1500 balls 4 balls per workgroup
+// and chunks of 128 partners; above, chunks of 64 and registers held to 3 wavefronts per
+// SIMD.
+int ordered_default_shape(int n) { return n < 1500 ? 0 : 2; }
+
+int ordered_blocks(int n, int shape)
 {
-    // the largest i-block that still gives about 940 blocks: the pass is latency-bound
-    // at a few wavefronts per SIMD, and smaller blocks raise that (measured best: 4 balls
-    // up to ~7,500 spheres, 8 up to ~15,000, 16 above; 343 vs 458 us per 5000-sphere step)
-    const int b = n >= 16 * 940 ? 16 : n >= 8 * 940 ? 8 : 4;
-    if (bi) *bi = b;
-    return (n + b - 1) / b;
+    const int bi = ORD_SHAPES[(shape >= 0 && shape < N_ORD_SHAPES) ? shape : 0].bi;
+    return (n + bi - 1) / bi;
 }
 
-hipError_t launch_pairs_ordered(const State& s, const Phys& P, int n, int write, double* pe_part,
-                                unsigned long long* contacts, hipStream_t st)
+hipError_t launch_pairs_ordered(const State& s, const Phys& P, int n, int write, double* pe_ball,
+                                unsigned long long* contacts, hipStream_t st, int shape)
 {
     if (n <= 1) return hipSuccess;
-    int bi = 0;
-    const int nb = ordered_blocks(n, &bi);
-    if (bi == 16)
-        k_pairs_ordered<16><<<nb, 256, 0, st>>>(s, P, n, write, pe_part, contacts);
-    else if (bi == 8)
-        k_pairs_ordered<8><<<nb, 256, 0, st>>>(s, P, n, write, pe_part, contacts);
-    else
-        k_pairs_ordered<4><<<nb, 256, 0, st>>>(s, P, n, write, pe_part, contacts);
+    if (shape < 0 || shape >= N_ORD_SHAPES) shape = 0;
+    const int nb = ordered_blocks(n, shape);
+    // template arguments: balls, producer wavefronts, partners per chunk, dedicated consumer
+    // wavefront, contact arm out of line, wavefronts per SIMD the registers are held to
+    switch (shape) {
+    case 0: k_pairs_ordered<4, 4, 128, false, false, 4><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
+    case 1: k_pairs_ordered<4, 2, 64, false, false, 0><<<nb, 128, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
+    case 2: k_pairs_ordered<4, 4, 64, false, false, 3><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
+    case 3: k_pairs_ordered<8, 4, 32, false, false, 3><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
+    case 4: k_pairs_ordered<2, 2, 128, false, false, 0><<<nb, 128, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
+    case 5: k_pairs_ordered<8, 4, 64, false, false, 0><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
+    case 6: k_pairs_ordered<2, 2, 128, false, true, 0><<<nb, 128, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
+    case 7: k_pairs_ordered<4, 4, 64, false, true, 0><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
+    default: k_pairs_ordered<8, 4, 32, false, true, 0><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
+    }
     return hipGetLastError();
 }
 

@@ -58,10 +58,42 @@ __device__ __forceinline__ double vdw_energy(double Ha, double Ra, double Rb, do
     return -Ha / 6 * (two_RaRb / ds + two_RaRb / dd + ::log(ds / dd));
 }
 
+// a / b, correctly rounded, from y = RN(1/b), the correctly rounded reciprocal (one
+// binary64 division per divisor): q0 = a y, then two corrections q <- q + (a - b q) y whose
+// residual a - b q an FMA forms exactly.  The first correction leaves q within one ulp of
+// a/b, so the second residual is exact and, y being within half an ulp of 1/b, the second
+// correction rounds to the correctly rounded quotient (Markstein's theorem; P. Markstein,
+// IBM J. Res. Dev. 34(1), 1990).  Valid while a, a/b and the residuals are normal numbers.
+// A zero quotient can come out +0 where a / b gives -0; every use here adds it to a sum
+// that started at +0, whose bits that does not change.
+__device__ __forceinline__ double div_rcp(double a, double b, double y)
+{
+    const double q0 = a * y;
+    const double q1 = __builtin_fma(__builtin_fma(-q0, b, a), y, q0);
+    return __builtin_fma(__builtin_fma(-q1, b, a), y, q1);
+}
+
 struct PairIn {
     D3 pA, pB;    // positions (after this step's drift)
     double RA, RB, mA, mB;
 };
+
+// Force on A of a pair that does not touch (overlap = S - dist <= 0) in the reference's
+// arithmetic (ColliderSingleCore.cpp:211-227, and gravity on all pairs as pair_ref adds
+// it): bitwise pair_ref's far arm, with the three divisions by dist taken from one
+// correctly rounded reciprocal (div_rcp).  h receives the clamped gap.
+__device__ __forceinline__ D3 far_force_ref(const Phys& P, double RA, double RB, double mA, double mB, const D3& r,
+                                            double dist, double overlap, double& h)
+{
+    h = ::fabs(overlap);                                          // :211-215
+    if (h < P.h_min) h = P.h_min;
+    const double fv = vdw_factor(P.Ha, RA, RB, h, h * h);
+    const double yd = 1.0 / dist;
+    const D3 rh = d3(div_rcp(r.x, dist, yd), div_rcp(r.y, dist, yd), div_rcp(r.z, dist, yd));   // r / norm(r)
+    D3 F = rh * fv;                                               // :221-225
+    if (P.far_gravity) F = F + rh * (P.G * mA * mB / (dist * dist));
+    return F;
+}
 
 struct PairOut {
     D3 F;         // force on A

@@ -1,6 +1,7 @@
 // Allocation-free stepping: the ordered (bitwise) step on state held in the caller's
-// device memory, with the step's scratch -- per-block energy partials, the touching-pair
-// counters and the non-finite flag -- in a fixed buffer of this library on each device.
+// device memory, with the step's scratch -- per-ball pair energies, per-block kinetic
+// partials, the touching-pair counters and the non-finite flag -- in a fixed buffer of
+// this library on each device.
 // Nothing here allocates or frees device memory, so the call can run where hipMalloc
 // may not: under HIP graph capture, in tight serving loops, in harnesses that forbid it.
 //
@@ -24,13 +25,13 @@ namespace {
 // Scratch layout, in doubles from the start of the buffer.  Sizes cover the largest
 // system the entry point accepts, SL_STATIC_MAX_BALLS balls.
 constexpr int ST_MAX = SL_STATIC_MAX_BALLS;
-constexpr int ST_PE_SLOTS = ST_MAX / 4 + 64;            // ordered pair-pass blocks (4 balls each at most)
+constexpr int ST_PE_SLOTS = ST_MAX + 64;                // pair energies, one per ball
 constexpr int ST_E_SLOTS = 7 * ((ST_MAX + 255) / 256 + 8);   // kick blocks x {KE, p, L}
 constexpr int ST_PE_OFF = 0;
 constexpr int ST_E_OFF = ST_PE_OFF + ST_PE_SLOTS;
 constexpr int ST_CNT_OFF = ST_E_OFF + ST_E_SLOTS;        // 2 x u64: contacts (running, this write step)
 constexpr int ST_FLAG_OFF = ST_CNT_OFF + 2;              // int: non-finite state seen
-constexpr int ST_DOUBLES = ST_FLAG_OFF + 2;               // 146 KB
+constexpr int ST_DOUBLES = ST_FLAG_OFF + 2;               // 540 KB
 
 // the caller's state layout, as the header documents it, is the kernels' own
 static_assert(NFIELD == SL_NFIELD && PX == SL_F_POS && VX == SL_F_VEL && WX == SL_F_W && AX == SL_F_ACC &&
@@ -38,8 +39,8 @@ static_assert(NFIELD == SL_NFIELD && PX == SL_F_POS && VX == SL_F_VEL && WX == S
                   MASS == SL_F_M && MOI == SL_F_MOI,
               "field order of sl_step_ordered_static");
 
-// Energies of a write step from the partials, summed in the order the context path
-// sums them on the host (ascending block index from +0.0), so the values are the same.
+// Energies of a write step from the partials, summed in the order the context path sums
+// them on the host (ascending ball or block index from +0.0), so the values are the same.
 __global__ void k_energy_out(const double* __restrict__ pe_part, int n_pe, const double* __restrict__ e_part,
                              int n_e, const unsigned long long* __restrict__ cnt, const int* __restrict__ flag,
                              double* __restrict__ out)
@@ -107,7 +108,8 @@ extern "C" int sl_step_ordered_static(const sl_params* p, int n, int stride, dou
     double* e_part = s + ST_E_OFF;
     unsigned long long* cnt = reinterpret_cast<unsigned long long*>(s + ST_CNT_OFF);
     int* flag = reinterpret_cast<int*>(s + ST_FLAG_OFF);
-    const int n_pe = n > 1 ? ordered_blocks(n, nullptr) : 0;
+    const int shape = ordered_default_shape(n);
+    const int n_pe = n > 1 ? n : 0;   // per-ball pair energies
     const int n_e = kick_blocks(n);
     if (n_pe > ST_PE_SLOTS || 7 * n_e > ST_E_SLOTS) return SL_E_CAPACITY;
 
@@ -133,7 +135,7 @@ extern "C" int sl_step_ordered_static(const sl_params* p, int n, int stride, dou
         const bool write = write_every > 0 && step % write_every == 0;
         if (launch_drift(S, n, p->dt, st) != hipSuccess) return SL_E_HIP;
         if (write && launch_zero_u64(cnt + 1, 1, st) != hipSuccess) return SL_E_HIP;
-        if (launch_pairs_ordered(S, P, n, write ? 1 : 0, pe_part, cnt, st) != hipSuccess) return SL_E_HIP;
+        if (launch_pairs_ordered(S, P, n, write ? 1 : 0, pe_part, cnt, st, shape) != hipSuccess) return SL_E_HIP;
         if (launch_kick(S, n, p->dt, write ? 1 : 0, 1, e_part, flag, st) != hipSuccess) return SL_E_HIP;
         if (write) {
             if (dev_energy) {

@@ -2,6 +2,7 @@
 // hipMemsetAsync so the same sequence is safe to capture in a HIP graph.
 #include "spacelab/sl_api.h"
 #include "sl_internal.hpp"
+#include "sl_pair_ref.hpp"
 
 namespace sl {
 
@@ -64,6 +65,39 @@ extern "C" int sl_probe_divsqrt(int n, const double* a, const double* b, double*
             hipMemcpy(q, d + 2 * (size_t)n, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(s, d + 3 * (size_t)n, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(lg, d + 4 * (size_t)n, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = SL_E_HIP;
+    }
+    hipFree(d);
+    return rc;
+}
+
+namespace sl {
+
+// The ordered kernels' division through a correctly rounded reciprocal.
+__global__ void k_probe_div_rcp(int n, const double* a, const double* b, double* q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double y = 1.0 / b[i];
+    q[i] = div_rcp(a[i], b[i], y);
+}
+
+}  // namespace sl
+
+extern "C" int sl_probe_div_rcp(int n, const double* a, const double* b, double* q)
+{
+    if (n < 1 || !a || !b || !q) return SL_E_ARG;
+    double* d = nullptr;
+    const size_t bytes = sizeof(double) * (size_t)n;
+    if (hipMalloc(&d, 3 * bytes) != hipSuccess) return SL_E_HIP;
+    int rc = SL_OK;
+    if (hipMemcpy(d, a, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d + n, b, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = SL_E_HIP;
+    } else {
+        sl::k_probe_div_rcp<<<(n + 255) / 256, 256>>>(n, d, d + n, d + 2 * (size_t)n);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(q, d + 2 * (size_t)n, bytes, hipMemcpyDeviceToHost) != hipSuccess)
             rc = SL_E_HIP;
     }
     hipFree(d);

@@ -52,6 +52,35 @@ def test_device_division_sqrt_correctly_rounded():
     assert np.max(np.abs(lg - np.log(a_pos)) / np.abs(np.log(a_pos))) < 1e-15
 
 
+def _adversarial_divisors(rng, n):
+    """Divisors whose significand is all ones or nearly (the hardest reciprocals), and
+    quotients that land on or next to exact values."""
+    e = rng.integers(-200, 200, n)
+    sig = (np.uint64((1 << 52) - 1) - rng.integers(0, 256, n).astype(np.uint64))
+    bits = ((e + 1023).astype(np.uint64) << np.uint64(52)) | sig
+    return bits.view(np.float64) * rng.choice([-1.0, 1.0], n)
+
+
+def test_device_division_by_reciprocal_correctly_rounded():
+    """The ordered kernels divide by a distance or a mass through the correctly rounded
+    reciprocal and two FMA residual corrections; bitwise parity needs the result to be
+    the correctly rounded quotient.  Random operands over 2^+-330, divisors with
+    all-ones significands, and quotients that are exact or one ulp off exact."""
+    rng = np.random.default_rng(12)
+    n = 1 << 20
+    a = np.exp(rng.uniform(-230, 230, n)) * rng.choice([-1.0, 1.0], n)
+    b = np.exp(rng.uniform(-230, 230, n)) * rng.choice([-1.0, 1.0], n)
+    assert bits_equal(sl.probe_div_rcp(a, b), a / b)
+    b2 = _adversarial_divisors(rng, n)
+    a2 = np.exp(rng.uniform(-200, 200, n)) * rng.choice([-1.0, 1.0], n)
+    assert bits_equal(sl.probe_div_rcp(a2, b2), a2 / b2)
+    # exact quotients and their neighbours
+    qx = np.round(rng.uniform(-1e6, 1e6, n)) * 2.0 ** rng.integers(-40, 40, n)
+    a3 = qx * b
+    a3 = np.where(rng.random(n) < 0.5, a3, np.nextafter(a3, np.inf))
+    assert bits_equal(sl.probe_div_rcp(a3, b), a3 / b)
+
+
 @pytest.mark.parametrize("name", golden_names())
 def test_golden_bitwise(name):
     z = load_golden(name)
@@ -217,3 +246,53 @@ def test_contact_count_at_write_steps():
             ew = c.step(3, 1)
         assert e0.contacts == k, order
         assert len(ew) == 3 and all(e.contacts == k for e in ew), (order, [e.contacts for e in ew])
+
+
+def _shape_count():
+    """Number of launch shapes of the ordered pass: the largest value the option accepts."""
+    g = sl.generate_bpca(8, seed=3)
+    p = sl.calibrate(sl.preset("config"), g.R, g.m, velocity=0.36301555459799423, cor=0.64)
+    with sl.Collider(8, p) as c:
+        k = 1
+        while True:
+            try:
+                c.set_option("ordered_shape", k + 1)
+            except sl.SpacelabError:
+                return k
+            k += 1
+            assert k < 64
+
+
+def test_ordered_shapes_bitwise(oracle_mod):
+    """Every launch shape of the ordered pass (balls per workgroup, chunk width, contact
+    arm inline or not, register cap) gives the oracle's bits: 700 spheres (a multiple of
+    none of the shapes' ball counts, several partial chunks), contacts from v_sigma = 5,
+    gravity on every pair, 30 steps with energies every 10."""
+    n = 700
+    g = sl.generate_bpca(n, seed=0x5EED0007, v_sigma=5.0)
+    p = sl.calibrate(sl.preset("config"), g.R, g.m, velocity=0.36301555459799423, cor=0.64)
+    st = oracle_mod.State(g.pos, g.vel, g.w, g.R, g.m, g.moi)
+    s, _, oew = oracle_mod.run(st, _oracle_params(oracle_mod, p), 31, 10, mode="gather",
+                               nthreads=NTHREADS)
+    shapes = _shape_count()
+    assert shapes >= 2
+    ref_e = None
+    for shape in range(1, shapes + 1):
+        with sl.Collider(n, p) as c:
+            c.set_option("ordered_shape", shape)
+            c.upload(g)
+            c.init_conditions()
+            ew = c.step(30, 10)
+            out = c.download()
+            assert c.get_option("ordered_shape") == shape
+        for k in ("pos", "vel", "velh", "w", "wh", "acc", "aacc"):
+            assert bits_equal(getattr(out, k), getattr(s, k)), (shape, k)
+        assert len(ew) == 3
+        e = np.array([[x.PE, x.KE, x.contacts] for x in ew])
+        if ref_e is None:
+            ref_e = e
+        # the energies are per-ball chains too: the same bits for every shape
+        assert np.array_equal(e.view(np.uint64), ref_e.view(np.uint64)), shape
+        for x, r in zip(ew, oew):
+            _energy_close(x, r, g.m, g.moi, out.pos, out.vel, out.w, rtol=1e-12)
+        assert ew[-1].contacts > 0
