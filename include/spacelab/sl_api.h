@@ -126,10 +126,13 @@ void sl_destroy(sl_ctx* ctx);
  *                      once as HIP graphs and replayed; 0: every kernel launched
  *                      directly.  Results are identical either way.
  *   SL_OPT_TILE_SPLIT  wavefronts per 64x64 pair tile: 0 (default) = chosen by the
- *                      built-in autotuner on first use, or 1, 2, 4.  The split changes
- *                      the summation order inside a tile, so runs compare bitwise only
- *                      at equal split.  sl_get_option reports the split in use (0
- *                      before the first fast step).
+ *                      built-in autotuner on first use of a configuration and stored in
+ *                      a JSON cache ($SPACELAB_TUNE_CACHE, else ~/.cache/spacelab_amd/
+ *                      tune.json; a shipped data/tune.json beside the library's lib/ is
+ *                      read too), so later runs of it use the same split; or 1, 2, 4.
+ *                      The split changes the summation order inside a tile, so runs
+ *                      compare bitwise only at equal split.  sl_get_option reports the
+ *                      split in use (0 before the first fast step).
  *   SL_OPT_BALL_WAVES  wavefronts per workgroup of the per-ball kernel: 0 (default) = 4
  *                      when the state has more 64-ball blocks than the device has
  *                      compute units and no system has more than 80 blocks (many small
@@ -319,6 +322,13 @@ int sl_collision_approach(int n_t, int n_p, double* pos3, const double* R, doubl
 #define SL_N_DESCRIPTORS 21
 int sl_collision_descriptors(int n_t, int n_p, const double* pos3, const double* R, const double* m,
                              double speed, double b_frac, double link, double* desc);
+/* The same for `count` collisions at once on up to `nthreads` host threads (0: one per
+ * core, at most 16).  Collision k's balls are first[k] .. first[k] + n_t[k] + n_p[k] - 1 of the
+ * concatenated pos3 (xyz per ball), R and m; its descriptors go to
+ * desc[k * SL_N_DESCRIPTORS ...].  Returns the first error of any collision, else SL_OK. */
+int sl_collision_descriptors_batch(int count, const int* n_t, const int* n_p, const long long* first,
+                                   const double* pos3, const double* R, const double* m, const double* speed,
+                                   const double* b_frac, double link, double* desc, int nthreads);
 
 /* ---- surrogate model (no reference counterpart) ---------------------------
  * A 4-layer perceptron n_in -> width -> width -> width -> n_out with ReLU between

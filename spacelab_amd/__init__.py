@@ -192,6 +192,9 @@ def lib():
     L.sl_csv_parse_energy_row.argtypes = [cp, C.c_int, dp]
     L.sl_collision_descriptors.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_double,
                                            C.c_double, dp]
+    L.sl_collision_descriptors_batch.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                 C.POINTER(C.c_longlong), dp, dp, dp, dp, dp, C.c_double, dp,
+                                                 C.c_int]
     fp = C.POINTER(C.c_float)
     L.sl_mlp_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong]
     L.sl_mlp_n_params.argtypes = [vp]
@@ -417,6 +420,31 @@ def collision_descriptors(setup: CollisionSetup, link: float) -> np.ndarray:
                                         _dp(_f64(g.m, (n,))), setup.speed, setup.b_frac, link, _dp(out))
     if rc:
         raise SpacelabError(rc, "sl_collision_descriptors")
+    return out
+
+
+def collision_descriptors_batch(setups: list, link: float, threads: int = 0) -> np.ndarray:
+    """collision_descriptors of many set-ups at once, on ``threads`` host threads (0: one
+    per core, at most 16); row k belongs to setups[k]."""
+    k = len(setups)
+    out = np.zeros((k, N_DESCRIPTORS))
+    if k == 0:
+        return out
+    sizes = np.array([s.group.num_particles for s in setups], dtype=np.int64)
+    first = np.ascontiguousarray(np.concatenate([[0], np.cumsum(sizes)[:-1]]), dtype=np.int64)
+    pos = np.ascontiguousarray(np.concatenate([_f64(s.group.pos, (s.group.num_particles, 3)) for s in setups]))
+    R = np.ascontiguousarray(np.concatenate([_f64(s.group.R, (s.group.num_particles,)) for s in setups]))
+    m = np.ascontiguousarray(np.concatenate([_f64(s.group.m, (s.group.num_particles,)) for s in setups]))
+    n_t = np.ascontiguousarray([s.n_t for s in setups], dtype=np.int32)
+    n_p = np.ascontiguousarray([s.n_p for s in setups], dtype=np.int32)
+    speed = np.ascontiguousarray([s.speed for s in setups], dtype=np.float64)
+    b = np.ascontiguousarray([s.b_frac for s in setups], dtype=np.float64)
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    rc = lib().sl_collision_descriptors_batch(k, ip(n_t), ip(n_p), first.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                              _dp(pos), _dp(R), _dp(m), _dp(speed), _dp(b), link, _dp(out),
+                                              int(threads))
+    if rc:
+        raise SpacelabError(rc, "sl_collision_descriptors_batch")
     return out
 
 

@@ -7,9 +7,11 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <numeric>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -354,6 +356,38 @@ int sl_collision_descriptors(int n_t, int n_p, const double* pos3, const double*
     }
     for (int k = 0; k < SL_N_DESCRIPTORS; k++) desc[k] = out[k];
     return SL_OK;
+}
+
+int sl_collision_descriptors_batch(int count, const int* n_t, const int* n_p, const long long* first,
+                                   const double* pos3, const double* R, const double* m, const double* speed,
+                                   const double* b_frac, double link, double* desc, int nthreads)
+{
+    if (count < 0) return SL_E_ARG;
+    if (count == 0) return SL_OK;
+    if (!n_t || !n_p || !first || !pos3 || !R || !m || !speed || !b_frac || !desc) return SL_E_ARG;
+    // collisions are handed out one at a time: their costs differ by orders of magnitude
+    std::atomic<int> next{0};
+    std::atomic<int> err{SL_OK};
+    auto work = [&] {
+        for (int k = next.fetch_add(1); k < count; k = next.fetch_add(1)) {
+            const long long o = first[k];
+            const int rc = o < 0 ? SL_E_ARG
+                                 : sl_collision_descriptors(n_t[k], n_p[k], pos3 + 3 * o, R + o, m + o, speed[k],
+                                                            b_frac[k], link, desc + (size_t)k * SL_N_DESCRIPTORS);
+            if (rc != SL_OK) {
+                int expected = SL_OK;
+                err.compare_exchange_strong(expected, rc);
+            }
+        }
+    };
+    int nt = nthreads > 0 ? nthreads : std::min(16, (int)std::thread::hardware_concurrency());
+    nt = std::max(1, std::min(nt, std::min(count, 64)));
+    std::vector<std::thread> pool;
+    pool.reserve(nt - 1);
+    for (int t = 1; t < nt; t++) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+    return err.load();
 }
 
 // Moves the projectile (balls n_t..) along x so that its first contact with the

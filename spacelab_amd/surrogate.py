@@ -33,7 +33,7 @@ import numpy as np
 
 from . import (N_DESCRIPTORS, OUTCOME_BOUNCE, OUTCOME_FRAGMENT, OUTCOME_STICK, ORDER_FAST,
                PREC_FP64, Batch, CollisionSetup, MLP, calibrate, collision_approach,
-               collision_descriptors, collision_outcome, generate_collision, preset)
+               collision_descriptors_batch, collision_outcome, generate_collision, preset)
 
 CLASS_NAMES = ("stick", "bounce", "fragment")
 # regression targets: mass fraction of the largest body, share of the projectile's mass
@@ -104,8 +104,11 @@ def prepare(spec: Spec, s: DemSettings) -> Prepared:
     return Prepared(setup, kin, kout, dt, met)
 
 
-def descriptors(preps: list[Prepared], s: DemSettings) -> np.ndarray:
-    return np.stack([collision_descriptors(q.setup, s.link * s.r_med) for q in preps]).astype(np.float32)
+def descriptors(preps: list[Prepared], s: DemSettings, threads: int = 0) -> np.ndarray:
+    """Descriptor rows of the set-ups, computed on ``threads`` host threads (0: all cores)."""
+    if not preps:
+        return np.zeros((0, N_DESCRIPTORS), dtype=np.float32)
+    return collision_descriptors_batch([q.setup for q in preps], s.link * s.r_med, threads).astype(np.float32)
 
 
 def targets(o, speed: float) -> np.ndarray:
@@ -343,6 +346,218 @@ def evaluate(train: dict, heldout: dict, target: float = 0.95, z: float = 2.0, v
     }
 
 
+# ---- look-ahead cascade -------------------------------------------------------------
+LOOK_AHEAD = 4000     # DEM steps of the cascade's look-ahead stage
+
+
+@dataclass
+class Cascade:
+    """Two entropy gates in a row.  Stage 1 reads a set-up's descriptors.  A collision
+    whose stage-1 entropy exceeds ``t1`` runs ``look`` DEM steps, and stage 2 reads the
+    descriptors plus the provisional outcome at that step (:func:`look_row`).  A
+    collision whose stage-2 entropy exceeds ``t2`` runs DEM to the end, from its set-up,
+    so that at a fixed tile split its result is the DEM-alone one bit for bit."""
+    s1: Surrogate
+    s2: Surrogate
+    look: int = LOOK_AHEAD
+    t1: float = float("inf")
+    t2: float = float("inf")
+
+    def close(self):
+        self.s1.close()
+        self.s2.close()
+
+    def calibrate(self, X, L, cls, target: float, look_cost: float) -> dict:
+        """Thresholds that reach ``target`` agreement on (X, L, cls) at the least DEM
+        cost per collision, look_cost x (stage-1 fallback share) + (share run to the
+        end), in units of one full DEM collision.  Stage-1 thresholds tried: the 201
+        quantiles of the stage-1 entropy and -1 (everything to stage 2); for each, the
+        stage-2 threshold is the loosest that reaches the target."""
+        p1, _, _, e1 = self.s1.predict(X)
+        p2, _, _, e2 = self.s2.predict(np.hstack([X, L]))
+        n = len(cls)
+        need = int(math.ceil(target * n - 1e-9))     # collisions that must agree
+        cands = np.unique(np.concatenate([np.quantile(e1, np.linspace(0, 1, 201)), [-1.0]]))
+        best = None
+        for t1 in cands:
+            F1 = e1 > t1
+            ok1 = int(((p1 == cls) & ~F1).sum())
+            idx = np.flatnonzero(F1)
+            o = idx[np.argsort(-e2[idx], kind="stable")]
+            right = p2[o] == cls[o]
+            # sending the k most uncertain of F1 to DEM: k right by definition, plus the
+            # stage-2 predictions that are right among the rest
+            right_after = np.append(right[::-1].cumsum()[::-1], 0)
+            hit = np.flatnonzero(ok1 + np.arange(len(o) + 1) + right_after >= need)
+            if len(hit) == 0:
+                continue
+            k = int(hit[0])
+            # e2 > t2 takes o[:k] and any ties with o[k-1]
+            t2 = float(np.nextafter(e2[o[k - 1]], -np.inf)) if k > 0 else float("inf")
+            F2 = F1 & (e2 > t2)
+            cost = look_cost * F1.mean() + F2.mean()
+            if best is None or cost < best[0] - 1e-12:
+                best = (float(cost), float(t1), t2)
+        if best is None:                                   # only DEM reaches the target
+            best = (1.0 + look_cost, -1.0, -1.0)
+        _, self.t1, self.t2 = best
+        F1 = e1 > self.t1
+        F2 = F1 & (e2 > self.t2)
+        final = np.where(F2, cls, np.where(F1, p2, p1))
+        return {"t1": self.t1, "t2": self.t2, "stage1_share": float(F1.mean()), "dem_share": float(F2.mean()),
+                "agreement": float((final == cls).mean()), "dem_cost": best[0]}
+
+
+def run_cascade(preps: list[Prepared], cas: Cascade, dem: "DemRunner", s: DemSettings):
+    """Outcome classes and regression targets of every collision through the cascade.
+    Returns (cls, reg, stage-1 fallback mask, run-to-the-end mask, look-ahead features
+    of the stage-1 fallbacks, timings in seconds)."""
+    n = len(preps)
+    t0 = time.perf_counter()
+    X = descriptors(preps, s)
+    t1 = time.perf_counter()
+    cls, reg, _, e1 = cas.s1.predict(X)
+    F1 = np.flatnonzero(e1 > cas.t1)
+    t2 = time.perf_counter()
+    m1 = np.zeros(n, dtype=bool)
+    m2 = np.zeros(n, dtype=bool)
+    L = np.zeros((len(F1), N_LOOK), dtype=np.float32)
+    t3 = t4 = t2
+    if len(F1):
+        m1[F1] = True
+        _, _, outs = dem.run([preps[i] for i in F1], steps=cas.look)
+        L = np.array([look_row(o, preps[i].setup.speed) for o, i in zip(outs, F1)],
+                     dtype=np.float32).reshape(-1, N_LOOK)
+        t3 = time.perf_counter()
+        c2, r2, _, e2 = cas.s2.predict(np.hstack([X[F1], L]))
+        cls[F1] = c2
+        reg[F1] = r2
+        F2 = F1[e2 > cas.t2]
+        t4 = time.perf_counter()
+        if len(F2):
+            m2[F2] = True
+            c, r, _ = dem.run([preps[i] for i in F2])
+            cls[F2] = c
+            reg[F2] = r
+    t5 = time.perf_counter()
+    return cls, reg, m1, m2, L, {"descriptors": t1 - t0, "stage1": t2 - t1, "look_dem": t3 - t2,
+                                 "stage2": t4 - t3, "dem": t5 - t4, "total": t5 - t0}
+
+
+def merge_datasets(parts: list[dict]) -> dict:
+    """Several datasets of the same settings as one (rows in order)."""
+    out = dict(parts[0])
+    for k, v in parts[0].items():
+        if k in ("settings", "sample_seed"):
+            continue
+        out[k] = np.concatenate([p[k] for p in parts])
+    for p in parts[1:]:
+        if str(p["settings"]) != str(parts[0]["settings"]):
+            raise ValueError("datasets of different DEM settings")
+    out["sample_seed"] = np.array([int(p["sample_seed"]) for p in parts])
+    return out
+
+
+def evaluate_cascade(train: dict, heldout: dict, target: float = 0.95, z: float = 2.0, val_frac: float = 0.5,
+                     epochs: int = 40, lr: float = 1e-3, n_models: int = 4, seed: int = 1,
+                     dem_all: bool = True, check: int = 512, log=None) -> dict:
+    """The cascade and the one-gate surrogate on the held-out collisions.
+
+    Both stages train on ``train`` with a validation share held back; the thresholds of
+    both gates are set on it, aiming above ``target`` by z standard errors of the
+    difference of two binomial estimates (validation and held-out), as :func:`evaluate`
+    does.  With ``dem_all`` every held-out collision also runs DEM alone (the reference
+    labels and the DEM-alone time); otherwise the stored DEM labels are the reference,
+    after the first ``check`` collisions run DEM alone and must reproduce them exactly
+    (the tile split is fixed, so a collision's result does not depend on its batch), and
+    the DEM-alone time is scaled from those.
+
+    The streaming gain counts per collision the surrogate's own time (descriptors and
+    both stages), the look-ahead share at look/steps of a full-batch DEM collision and
+    the run-to-the-end share at one: fallbacks of a long stream fill full batches."""
+    s = settings_of(heldout)
+    n = len(train["cls"])
+    perm = np.random.default_rng(seed).permutation(n)
+    nv = int(n * val_frac)
+    va, tr = perm[:nv], perm[nv:]
+    look_key = f"L{LOOK_AHEAD}"
+    X, C, Rg, Lf = train["X"], train["cls"], train["reg"], train[look_key]
+    nh = len(heldout["cls"])
+    margin = z * math.sqrt(target * (1 - target) * (1.0 / max(nv, 1) + 1.0 / nh))
+    look_cost = LOOK_AHEAD / s.steps
+    cas = Cascade(Surrogate(seed=seed, n_models=n_models),
+                  Surrogate(seed=seed + 100, n_models=n_models, n_in=N_DESCRIPTORS + N_LOOK))
+    dem = None
+    try:
+        t0 = time.perf_counter()
+        cas.s1.fit(X[tr], C[tr], Rg[tr], epochs=epochs, lr=lr, seed=seed)
+        cas.s2.fit(np.hstack([X[tr], Lf[tr]]), C[tr], Rg[tr], epochs=epochs, lr=lr, seed=seed + 100)
+        t_train = time.perf_counter() - t0
+        gate1 = cas.s1.calibrate_gate(X[va], C[va], min(target + margin, 1.0))
+        gate2 = cas.calibrate(X[va], Lf[va], C[va], min(target + margin, 1.0), look_cost)
+        if log:
+            log(f"trained two stages in {t_train:.1f} s; one gate {gate1}; cascade {gate2}")
+        specs = specs_of(heldout)
+        preps = [prepare(q, s) for q in specs]
+        max_balls = s.max_systems * max(q.setup.group.num_particles for q in preps)
+        dem = DemRunner(s, max_balls)
+        dem.run(preps[:16], steps=256)          # first-use set-up (tile-split tuning, graphs)
+        ref = heldout["cls"].astype(np.int32)
+        if dem_all:
+            t0 = time.perf_counter()
+            c_dem, r_dem, _ = dem.run(preps)
+            t_dem = time.perf_counter() - t0
+            n_timed = nh
+        else:
+            n_timed = min(check, nh)
+            t0 = time.perf_counter()
+            c_dem, r_dem, _ = dem.run(preps[:n_timed])
+            t_dem = time.perf_counter() - t0
+        stored_match = float((c_dem == ref[:len(c_dem)]).mean())
+        if dem_all:
+            ref = c_dem
+        c_c, r_c, m1, m2, Lh, tc = run_cascade(preps, cas, dem, s)
+        Xh = descriptors(preps, s)
+        p1, _, _, e1 = cas.s1.predict(Xh)
+        look_diff = float(np.abs(Lh - heldout[look_key][m1]).max()) if m1.any() else 0.0
+        # provisional class: the one-hot columns of the look-ahead row
+        look_cls = float((Lh[:, :3].argmax(1) == heldout[look_key][m1][:, :3].argmax(1)).mean()) if m1.any() else 1.0
+    finally:
+        cas.close()
+        if dem is not None:
+            dem.close()
+    c_d = t_dem / n_timed                                     # DEM seconds per collision, full batches
+    c_s = (tc["descriptors"] + tc["stage1"] + tc["stage2"]) / nh
+    fb1 = e1 > cas.s1.threshold
+    one = np.where(fb1, ref, p1)
+    f1, f2 = float(m1.mean()), float(m2.mean())
+    keep = ~m2
+    out = {
+        "n_train": int(len(tr)), "n_val": int(nv), "n_heldout": nh, "epochs": epochs, "n_models": n_models,
+        "train_s": t_train, "gate_target": target + margin, "look_ahead_steps": LOOK_AHEAD,
+        "dem_steps": s.steps, "softening": s.softening,
+        "raw_accuracy": float((p1 == ref).mean()),
+        "agreement": float((c_c == ref).mean()), "stage1_share": f1, "dem_share": f2,
+        "gain_streaming": c_d / (c_s + f1 * look_cost * c_d + f2 * c_d),
+        "surrogate_ms_per_collision": c_s * 1e3, "dem_ms_per_collision": c_d * 1e3,
+        "cascade_s": tc["total"], "cascade_parts_s": tc,
+        "one_gate": {"agreement": float((one == ref).mean()), "fallback_share": float(fb1.mean()),
+                     "gain_streaming": c_d / (c_s + float(fb1.mean()) * c_d)},
+        "val_one_gate": gate1, "val_cascade": gate2,
+        "dem_alone_collisions": n_timed, "dem_labels_match_stored": stored_match,
+        "look_features_max_abs_diff_vs_stored": look_diff, "look_class_match_stored": look_cls,
+        "heldout_classes": np.bincount(ref, minlength=3).tolist(),
+    }
+    if dem_all:
+        out["dem_s"] = t_dem
+        out["gain"] = t_dem / tc["total"]
+        out["collisions_per_s_dem"] = nh / t_dem
+        out["collisions_per_s_cascade"] = nh / tc["total"]
+        out["reg_mse"] = float(((r_c[keep] - r_dem[keep]) ** 2).mean()) if keep.any() else 0.0
+        out["reg_mse_mean_predictor"] = float(((Rg[tr].mean(0) - r_dem[keep]) ** 2).mean()) if keep.any() else 0.0
+    return out
+
+
 N_T_RANGE, N_P_RANGE, SPEED_RANGE = (32, 512), (1, 64), (10.0, 1000.0)
 
 
@@ -419,6 +634,16 @@ def look_row(o, speed: float) -> np.ndarray:
 def load_dataset(path: str) -> dict:
     with np.load(path, allow_pickle=False) as f:
         return {k: f[k] for k in f.files}
+
+
+HELDOUT_FILES = ("collisions_heldout.npz", "collisions_heldout_b.npz")   # seeds 2 and 3
+
+
+def load_heldout(data_dir: str) -> dict:
+    """The held-out collisions: the committed held-out files of ``data_dir`` as one set
+    (2,000 from sample seed 2, then 8,000 from sample seed 3)."""
+    import os
+    return merge_datasets([load_dataset(os.path.join(data_dir, f)) for f in HELDOUT_FILES])
 
 
 def main(argv=None):

@@ -170,3 +170,20 @@ def test_descriptors_impact_geometry():
     assert d[12] == 1 and d[16] == pytest.approx(1.0)
     assert d[19] == 1 and d[20] == 0    # the end ball of the chain touches one neighbour
     assert d[15] == pytest.approx(np.log(1 + n))
+
+
+def test_descriptors_batch_equals_one_at_a_time():
+    """The threaded batch entry gives every collision's descriptors exactly as the
+    single call does, whatever the thread count (sizes from a lone ball to hundreds)."""
+    setups = []
+    for k, (nt, npj) in enumerate([(1, 1), (300, 40), (32, 1), (120, 64), (7, 3), (512, 2)]):
+        s = sl.generate_collision(nt, npj, seed=100 + k, speed=50.0 * (k + 1), b_frac=0.1 * k)
+        sl.collision_approach(s, 5e-7)
+        setups.append(s)
+    link = 1e-6
+    one = np.stack([sl.collision_descriptors(s, link) for s in setups])
+    for threads in (0, 1, 3, 16):
+        assert np.array_equal(sl.collision_descriptors_batch(setups, link, threads), one), threads
+    assert sl.collision_descriptors_batch([], link).shape == (0, sl.N_DESCRIPTORS)
+    with pytest.raises(sl.SpacelabError):
+        sl.collision_descriptors_batch(setups, -1.0)

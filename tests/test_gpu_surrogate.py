@@ -43,3 +43,25 @@ def test_surrogate_gain_and_agreement():
     assert r["reg_mse"] < 0.5 * r["reg_mse_mean_predictor"]
     # and it is faster than DEM alone
     assert r["gain"] > 2.0 and r["gain_streaming"] > r["gain"] * 0.9
+
+
+def test_cascade_gain_and_agreement():
+    """The look-ahead cascade on the 10,000 held-out collisions, against their stored DEM
+    labels.  The first 512 collisions rerun DEM alone and must reproduce the stored
+    labels (up to the few per cent of chaotic outcomes that flip when the engine's
+    summation order changes in the last bit: 6 of 512 between two engine versions), and
+    the look-ahead stage's provisional classes must be the stored ones likewise.  The
+    cascade then has to agree with DEM on at least 95% of the collisions and cost less
+    DEM per collision than the one-gate surrogate at the same bar."""
+    train = S.load_dataset(os.path.join(DATA, "collisions_train.npz"))
+    held = S.load_heldout(DATA)
+    r = S.evaluate_cascade(train, held, target=0.95, dem_all=False, check=512,
+                           log=lambda m: print(m, flush=True))
+    print(json.dumps(r, indent=1, default=float), flush=True)
+    assert r["n_heldout"] == 10000
+    assert r["dem_labels_match_stored"] >= 0.97
+    assert r["look_class_match_stored"] >= 0.97
+    assert r["agreement"] >= 0.95
+    assert r["one_gate"]["agreement"] >= 0.95
+    assert r["gain_streaming"] > r["one_gate"]["gain_streaming"]
+    assert r["raw_accuracy"] >= 0.8
