@@ -125,3 +125,22 @@ def test_binding_constants_match_header():
     assert (defs["SL_OUTCOME_STICK"], defs["SL_OUTCOME_BOUNCE"], defs["SL_OUTCOME_FRAGMENT"]) == (
         sl.OUTCOME_STICK, sl.OUTCOME_BOUNCE, sl.OUTCOME_FRAGMENT)
     assert defs["SL_N_DESCRIPTORS"] == sl.N_DESCRIPTORS
+
+
+def test_shipped_tune_cache_is_valid():
+    """The tune cache shipped beside the library: a flat JSON object whose entries name a
+    tile split (1, 2 or 4) or an ordered launch shape (1 + index, at most 64), with the
+    timings they were chosen from."""
+    import json
+    path = os.path.join(os.path.dirname(sl.__file__), "data", "tune.json")
+    data = json.load(open(path))
+    assert data
+    for key, v in data.items():
+        kind = key.split(":")[0]
+        assert kind in ("tiles", "ordered"), key
+        if kind == "tiles":
+            assert v["split"] in (1, 2, 4), key
+            assert len(v["ms"]) == 3
+        else:
+            assert 1 <= v["split"] <= 64, key
+            assert len(v["ms"]) >= 2

@@ -360,3 +360,73 @@ def test_benchmark_aggregate_5000_fast_and_mixed(oracle_mod):
     _, e0m, ewm = _run(g, p.replace(precision=sl.PREC_MIXED), 100, 50)
     for a, b in zip([e0m] + ewm, [e0] + ew):
         assert abs(a.E - b.E) <= 1e-6 * (abs(b.PE) + b.KE)
+
+
+def test_mixed_against_oracle_5000(oracle_mod):
+    """The mixed path against the oracle (the reference's single-core arithmetic) on the
+    benchmark's own workload, the 5000-sphere aggregate: after one step the
+    accelerations to 1e-6 (binary32 far field, binary64 contacts; measured 2.0e-7, spins
+    3.2e-8), after 100 steps the positions to 1e-6 (measured 1.8e-7) and velocities to
+    2e-3 (4.4e-4: the one-step difference grows with the aggregate's e-folding time of
+    about five steps) of their largest values, and the energies at the write steps to
+    1e-6 of the energy scale (1.2e-10 and 1.4e-8)."""
+    n = 5000
+    g = sl.generate_bpca(n, seed=0x5EED0001, v_sigma=1e-3)
+    p = sl.calibrate(sl.preset("config"), g.R, g.m, velocity=V_CUSTOM, cor=0.64, dt_rule="tc50").replace(
+        order=sl.ORDER_FAST, precision=sl.PREC_MIXED)
+    P = _oracle_params(oracle_mod, p)
+    st = oracle_mod.State(g.pos, g.vel, g.w, g.R, g.m, g.moi)
+    out1, e0, _ = _run(g, p, 1)
+    s1, oe0, _ = oracle_mod.run(st, P, 2, 0, mode="gather", nthreads=NTHREADS)
+    print("one step: acc", relerr(out1.acc, s1.acc), "aacc", relerr(out1.aacc, s1.aacc))
+    assert relerr(out1.acc, s1.acc) <= 1e-6
+    assert relerr(out1.aacc, s1.aacc) <= 1e-6
+    assert abs(e0.PE - oe0[0]) <= 1e-6 * abs(oe0[0])
+    out, _, ew = _run(g, p, 100, 50)
+    s, _, oew = oracle_mod.run(st, P, 101, 50, mode="gather", nthreads=NTHREADS)
+    print("100 steps: pos", relerr(out.pos, s.pos), "vel", relerr(out.vel, s.vel),
+          "E", [abs(e.E - (r[0] + r[1])) / (abs(r[0]) + r[1]) for e, r in zip(ew, oew)])
+    assert relerr(out.pos, s.pos) <= 1e-6
+    assert relerr(out.vel, s.vel) <= 2e-3
+    for e, r in zip(ew, oew):
+        assert abs(e.E - (r[0] + r[1])) <= 1e-6 * (abs(r[0]) + r[1])
+
+
+_REPEAT_JOB = r'''
+import hashlib, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import spacelab_amd as sl
+g = sl.generate_bpca(700, seed=5, v_sigma=3.0)
+p = sl.calibrate(sl.preset("config"), g.R, g.m, velocity=0.36301555459799423, cor=0.64).replace(
+    order=sl.ORDER_FAST)
+with sl.Collider(700, p) as c:
+    c.upload(g)
+    c.init_conditions()
+    c.step(50, 0)
+    st = c.download()
+    split = c.get_option("tile_split")
+h = hashlib.sha256(np.concatenate([st.pos.ravel(), st.vel.ravel(), st.w.ravel()]).tobytes()).hexdigest()
+print(split, h)
+'''
+
+
+def test_default_runs_repeat_bitwise_through_tune_cache(tmp_path):
+    """Two processes run the same fast-path job with the tile split left to the
+    autotuner: the first tunes and stores its choice in the tune cache, the second reads
+    it back, so the two repeat bit for bit without forcing a split."""
+    import json
+    import subprocess
+    import sys
+    env = dict(os.environ, SPACELAB_TUNE_CACHE=str(tmp_path / "tune.json"))
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    outs = []
+    for _ in range(2):
+        r = subprocess.run([sys.executable, "-c", _REPEAT_JOB, repo], env=env, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        outs.append(r.stdout.split())
+    assert outs[0] == outs[1]
+    cache = json.loads((tmp_path / "tune.json").read_text())
+    tiles = [v["split"] for k, v in cache.items() if k.startswith("tiles:prec0:")]
+    assert tiles == [int(outs[0][0])]
