@@ -165,7 +165,7 @@ __device__ __attribute__((noinline)) void contact_pair(const double* base, long 
 
 // CALL: the contact arm out of line (its registers weigh only on the calls) or inline.
 // WPE: the wavefronts per SIMD the register budget is held to (0: the compiler's choice).
-template <int BI, int NPW, int CJ, bool DED, bool CALL, int WPE>
+template <int BI, int NPW, int CJ, bool DED, bool CALL, int WPE, int UNR = 1>
 __global__ __launch_bounds__((NPW + (DED ? 1 : 0)) * 64) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_pairs_ordered(State s, Phys P, int n, int write, double* __restrict__ pe_ball,
                      unsigned long long* __restrict__ contacts)
@@ -332,11 +332,11 @@ void k_pairs_ordered(State s, Phys P, int n, int write, double* __restrict__ pe_
             };
             if (j0 + CJ <= i0) {
                 // every partner of the chunk precedes every ball of the block: i is A
-#pragma unroll 1
+#pragma unroll UNR
                 for (int q = 0; q < KP; q++) far_pair(std::true_type{}, q);
             } else if (j0 >= i0 + BI) {
                 // every partner follows: i is B
-#pragma unroll 1
+#pragma unroll UNR
                 for (int q = 0; q < KP; q++) far_pair(std::false_type{}, q);
             } else {
 #pragma unroll 1
@@ -467,7 +467,7 @@ struct OrdShape {
 };
 constexpr OrdShape ORD_SHAPES[] = {
     {4, 4, 128, 0}, {4, 2, 64, 0}, {4, 4, 64, 0}, {8, 4, 32, 0}, {2, 2, 128, 0},
-    {8, 4, 64, 0},  {2, 2, 128, 0}, {4, 4, 64, 0}, {8, 4, 32, 0},
+    {8, 4, 64, 0},  {2, 2, 128, 0}, {8, 4, 64, 0}, {4, 2, 64, 0},
 };
 constexpr int N_ORD_SHAPES = (int)(sizeof(ORD_SHAPES) / sizeof(ORD_SHAPES[0]));
 }  // namespace
@@ -502,8 +502,8 @@ hipError_t launch_pairs_ordered(const State& s, const Phys& P, int n, int write,
     case 4: k_pairs_ordered<2, 2, 128, false, false, 0><<<nb, 128, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
     case 5: k_pairs_ordered<8, 4, 64, false, false, 0><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
     case 6: k_pairs_ordered<2, 2, 128, false, true, 0><<<nb, 128, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
-    case 7: k_pairs_ordered<4, 4, 64, false, true, 0><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
-    default: k_pairs_ordered<8, 4, 32, false, true, 0><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
+    case 7: k_pairs_ordered<8, 4, 64, false, false, 0, 2><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
+    default: k_pairs_ordered<4, 2, 64, false, false, 0, 2><<<nb, 128, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
     }
     return hipGetLastError();
 }
