@@ -148,6 +148,12 @@ void sl_destroy(sl_ctx* ctx);
  *                      results bit for bit.  sl_get_option reports the shape in use (0
  *                      before the first step). */
 #define SL_OPT_ORDERED_SHAPE 4
+/*   SL_OPT_TILE_QUEUE  how the fast engine's tile kernel is launched: 0 (default) = chosen by
+ *                      the autotuner with the split, 1 = one workgroup per tile group, 2 =
+ *                      as many workgroups as the device holds at once pulling tiles from a
+ *                      work queue.  Results are identical either way.  sl_get_option reports
+ *                      1 or 2 for the launch in use (0 before the first fast step). */
+#define SL_OPT_TILE_QUEUE 5
 int sl_set_option(sl_ctx* ctx, int option, int value);
 int sl_get_option(sl_ctx* ctx, int option, int* value);
 

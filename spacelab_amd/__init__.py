@@ -30,8 +30,9 @@ PREC_FP64, PREC_MIXED = 0, 1
 ORDER_ORDERED, ORDER_FAST = 0, 1
 OPT_GRAPHS, OPT_TILE_SPLIT, OPT_BALL_WAVES = 1, 2, 3
 OPT_ORDERED_SHAPE = 4
+OPT_TILE_QUEUE = 5
 _OPTS = {"graphs": OPT_GRAPHS, "tile_split": OPT_TILE_SPLIT, "ball_waves": OPT_BALL_WAVES,
-         "ordered_shape": OPT_ORDERED_SHAPE}
+         "ordered_shape": OPT_ORDERED_SHAPE, "tile_queue": OPT_TILE_QUEUE}
 
 ERRORS = {-1: "SL_E_ARG", -2: "SL_E_HIP", -3: "SL_E_STATE", -4: "SL_E_CAPACITY", -5: "SL_E_NOGPU"}
 
@@ -551,7 +552,8 @@ class Collider:
 
     def set_option(self, name: str, value: int):
         """'graphs' (0/1), 'tile_split' (0 = autotune, 1, 2, 4) or 'ball_waves' (0 = by
-        size, 4, 8) of the fast engine; 'ordered_shape' (0 = autotune, 1..n) of the
+        size, 4, 8) and 'tile_queue' (0 = autotune, 1 = one workgroup per tile group, 2 =
+        work queue) of the fast engine; 'ordered_shape' (0 = autotune, 1..n) of the
         ordered pair pass."""
         self._check(lib().sl_set_option(self._ctx, _OPTS[name], int(value)), "sl_set_option")
 

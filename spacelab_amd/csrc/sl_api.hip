@@ -411,6 +411,13 @@ int sl_set_option(sl_ctx* c, int option, int value)
         c->fb.split = 0;
         fast_clear_graphs(c->fb);
         return SL_OK;
+    case SL_OPT_TILE_QUEUE:
+        if (value < 0 || value > 2)
+            return set_err(c, SL_E_ARG, "tile queue must be 0 (autotune), 1 (per workgroup) or 2 (work queue) (got %d)", value);
+        c->fb.forced_queue = value;
+        c->fb.split = 0;
+        fast_clear_graphs(c->fb);
+        return SL_OK;
     case SL_OPT_BALL_WAVES:
         if (value != 0 && value != 4 && value != 8)
             return set_err(c, SL_E_ARG, "ball waves must be 0 (by size), 4 or 8 (got %d)", value);
@@ -437,6 +444,9 @@ int sl_get_option(sl_ctx* c, int option, int* value)
         return SL_OK;
     case SL_OPT_TILE_SPLIT:
         *value = c->fb.split;
+        return SL_OK;
+    case SL_OPT_TILE_QUEUE:
+        *value = c->fb.split == 0 ? 0 : (c->fb.queue_launch ? 2 : 1);
         return SL_OK;
     case SL_OPT_BALL_WAVES:
         *value = fast_ball_waves(c->fb);

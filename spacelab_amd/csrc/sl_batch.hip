@@ -264,6 +264,13 @@ int sl_batch_set_option(sl_batch* b, int option, int value)
         b->fb.split = 0;
         fast_clear_graphs(b->fb);
         return SL_OK;
+    case SL_OPT_TILE_QUEUE:
+        if (value < 0 || value > 2)
+            return berr(b, SL_E_ARG, "tile queue must be 0 (autotune), 1 (per workgroup) or 2 (work queue) (got %d)", value);
+        b->fb.forced_queue = value;
+        b->fb.split = 0;
+        fast_clear_graphs(b->fb);
+        return SL_OK;
     case SL_OPT_BALL_WAVES:
         if (value != 0 && value != 4 && value != 8)
             return berr(b, SL_E_ARG, "ball waves must be 0 (by size), 4 or 8 (got %d)", value);
@@ -284,6 +291,9 @@ int sl_batch_get_option(sl_batch* b, int option, int* value)
         return SL_OK;
     case SL_OPT_TILE_SPLIT:
         *value = b->fb.split;
+        return SL_OK;
+    case SL_OPT_TILE_QUEUE:
+        *value = b->fb.split == 0 ? 0 : (b->fb.queue_launch ? 2 : 1);
         return SL_OK;
     case SL_OPT_BALL_WAVES:
         *value = fast_ball_waves(b->fb);

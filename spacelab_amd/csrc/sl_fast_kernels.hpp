@@ -118,6 +118,7 @@ struct TileArgs {
     double* part;           // [slot][xyz][n_pad]
     unsigned long long* nbm;   // [slot][n_pad]: near-partner masks
     double* pe_tile;        // per tile: far-field pair energy (write steps)
+    int* queue;             // k_tiles_q: next group per segment, workgroups done (all 0 between launches)
 };
 
 // Sum over the 64 lanes of a wavefront as a fixed tree (deterministic).
@@ -240,7 +241,7 @@ __device__ __forceinline__ double3 block_ref(double4 lo, double4 hi)
 // j-side sums and masks once, to slot J of the balls of block I and slot I of the
 // balls of block J, and its far-pair energy on write steps.
 template <int PREC, bool WRITE, bool GRAV, int SPLIT>
-__global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
+__device__ __forceinline__ void tile_pass(const TileArgs& a, int tbase)
 {
     constexpr int TPB = NW / SPLIT;
     constexpr size_t SJ_BYTES = sizeof(JRec64) * TPB * 2 * TB;
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
     // derived from it are too
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / TB), l = threadIdx.x % TB;
     const int tl = w / SPLIT, h = w % SPLIT;
-    const int t = blockIdx.x * TPB + tl;
+    const int t = tbase + tl;
     const bool live = t < a.n_tiles;
     const int4 tt = fetch_tile(a, t, live);
     const int I = tt.x, J = tt.y, sys = tt.z, blk0 = tt.w;
@@ -583,6 +584,50 @@ __global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
         double pt = pe_w[w0];
         for (int q = 1; q < SPLIT; q++) pt = pt + pe_w[w0 + q];
         a.pe_tile[t] = pt;
+    }
+}
+
+// One launch, one workgroup per TPB tiles of the list.
+template <int PREC, bool WRITE, bool GRAV, int SPLIT>
+__global__ __launch_bounds__(NT) void k_tiles(TileArgs a)
+{
+    tile_pass<PREC, WRITE, GRAV, SPLIT>(a, blockIdx.x * (NW / SPLIT));
+}
+
+// The same tiles from a work queue: as many workgroups as fit on the device at once, each
+// taking the next group of TPB tiles until none is left.  The list is cut into QSEG
+// segments with a counter each, QPAD ints apart (device-scope atomics on one cache line
+// serialise, some 10-30 ns apiece); workgroup b starts on segment b mod QSEG and, when
+// that is empty, takes from the others in turn.  Every tile runs the same code as in
+// k_tiles, so the results are the same bits.  queue[QPAD s] is segment s's next group,
+// queue[QPAD QSEG] counts the workgroups done; the last one rewinds them all for the next
+// launch, after every workgroup has taken its last item.
+constexpr int QSEG = 8;
+constexpr int QPAD = 64;   // 256 bytes between counters
+template <int PREC, bool WRITE, bool GRAV, int SPLIT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WRITE ? 1 : 6))) void k_tiles_q(TileArgs a)
+{
+    constexpr int TPB = NW / SPLIT;
+    __shared__ int next;
+    const int groups = (a.n_tiles + TPB - 1) / TPB;
+    const int seg0 = blockIdx.x % QSEG;
+    for (int k = 0; k < QSEG; k++) {
+        const int seg = (seg0 + k) % QSEG;
+        const int g0 = (int)((long long)groups * seg / QSEG), g1 = (int)((long long)groups * (seg + 1) / QSEG);
+        for (;;) {
+            if (threadIdx.x == 0) next = g0 + atomicAdd(a.queue + QPAD * seg, 1);
+            __syncthreads();
+            const int g = __builtin_amdgcn_readfirstlane(next);   // uniform: scalar registers
+            __syncthreads();   // every thread has read `next` before it is written again
+            if (g >= g1) break;
+            tile_pass<PREC, WRITE, GRAV, SPLIT>(a, g * TPB);
+            __syncthreads();   // the pass's LDS is reused by the next item
+        }
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(a.queue + QPAD * QSEG, 1) == (int)gridDim.x - 1)
+            for (int q = 0; q <= QSEG; q++) atomicExch(a.queue + QPAD * q, 0);
     }
 }
 

@@ -102,6 +102,9 @@ struct FastBuffers {
     double* h_pe_tile = nullptr;
     int precision = 0;
     int split = 0;                // wavefronts per 64x64 tile (0: not tuned yet)
+    bool queue_launch = false;    // tiles from the work queue (k_tiles_q) rather than one per workgroup
+    int forced_queue = 0;         // 1 forces the per-workgroup launch, 2 the work queue
+    int* queue = nullptr;         // k_tiles_q's per-segment next groups and finished workgroups (0 between launches)
     int split_key = -1;           // (precision, gravity) the choice was tuned for
     int forced_split = 0;         // 1, 2 or 4 forces the choice
     int n_cu = 256;               // compute units of the device (k_ball's shape)

@@ -133,37 +133,66 @@ int err_hip(std::string* err, const char* what)
     return SL_E_HIP;
 }
 
+// One workgroup per TPB tiles, or (queue) as many workgroups as the device holds at once
+// pulling tiles from the work queue.  Both give the same bits.
 template <int PREC, bool WRITE, bool GRAV, int SPLIT>
-void launch_far(const TileArgs& a, hipStream_t st)
+void launch_far(const TileArgs& a, bool queue, int n_cu, hipStream_t st)
 {
     constexpr int TPB = NW / SPLIT;
-    k_tiles<PREC, WRITE, GRAV, SPLIT><<<(a.n_tiles + TPB - 1) / TPB, NT, 0, st>>>(a);
+    const int blocks = (a.n_tiles + TPB - 1) / TPB;
+    if (!queue) {
+        k_tiles<PREC, WRITE, GRAV, SPLIT><<<blocks, NT, 0, st>>>(a);
+        return;
+    }
+    // workgroups one CU holds at once, from the kernel's registers and LDS (512 VGPRs per
+    // SIMD lane in granules of 8, at most 8 wavefronts per SIMD, 4 SIMDs and 160 KB of
+    // LDS per CU)
+    static const int per_cu = [] {
+        hipFuncAttributes fa{};
+        if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_tiles_q<PREC, WRITE, GRAV, SPLIT>)) !=
+            hipSuccess)
+            return 1;
+        const int vg = std::max(8, (fa.numRegs + 7) / 8 * 8);
+        const int waves_per_simd = std::min(8, 512 / vg);
+        const int by_regs = waves_per_simd * 4 / (NT / 64);
+        const int by_lds = fa.sharedSizeBytes > 0 ? (int)(163840 / fa.sharedSizeBytes) : 64;
+        return std::max(1, std::min(by_regs, by_lds));
+    }();
+    const int grid = std::max(1, std::min(blocks, per_cu * n_cu));
+    k_tiles_q<PREC, WRITE, GRAV, SPLIT><<<grid, NT, 0, st>>>(a);
 }
 
 template <int SPLIT>
-void launch_far_split(const TileArgs& a, int precision, bool write, bool grav, hipStream_t st)
+void launch_far_split(const TileArgs& a, int precision, bool write, bool grav, bool q, int n_cu, hipStream_t st)
 {
     if (precision == SL_PREC_MIXED) {
         if (write)
-            grav ? launch_far<SL_PREC_MIXED, true, true, SPLIT>(a, st) : launch_far<SL_PREC_MIXED, true, false, SPLIT>(a, st);
+            grav ? launch_far<SL_PREC_MIXED, true, true, SPLIT>(a, q, n_cu, st)
+                 : launch_far<SL_PREC_MIXED, true, false, SPLIT>(a, q, n_cu, st);
         else
-            grav ? launch_far<SL_PREC_MIXED, false, true, SPLIT>(a, st) : launch_far<SL_PREC_MIXED, false, false, SPLIT>(a, st);
+            grav ? launch_far<SL_PREC_MIXED, false, true, SPLIT>(a, q, n_cu, st)
+                 : launch_far<SL_PREC_MIXED, false, false, SPLIT>(a, q, n_cu, st);
     } else {
         if (write)
-            grav ? launch_far<SL_PREC_FP64, true, true, SPLIT>(a, st) : launch_far<SL_PREC_FP64, true, false, SPLIT>(a, st);
+            grav ? launch_far<SL_PREC_FP64, true, true, SPLIT>(a, q, n_cu, st)
+                 : launch_far<SL_PREC_FP64, true, false, SPLIT>(a, q, n_cu, st);
         else
-            grav ? launch_far<SL_PREC_FP64, false, true, SPLIT>(a, st) : launch_far<SL_PREC_FP64, false, false, SPLIT>(a, st);
+            grav ? launch_far<SL_PREC_FP64, false, true, SPLIT>(a, q, n_cu, st)
+                 : launch_far<SL_PREC_FP64, false, false, SPLIT>(a, q, n_cu, st);
     }
 }
 
-void launch_far_any(const TileArgs& a, int split, int precision, bool write, bool grav, hipStream_t st)
+// choice: the tile split (1, 2, 4), plus 8 for the work-queue launch
+void launch_far_any(const TileArgs& a, int choice, int precision, bool write, bool grav, int n_cu, hipStream_t st)
 {
+    const int split = choice & 7;
+    const bool q = (choice & 8) != 0;
     if (split == 4)
-        launch_far_split<4>(a, precision, write, grav, st);
+        launch_far_split<4>(a, precision, write, grav, q, n_cu, st);
     else if (split == 2)
-        launch_far_split<2>(a, precision, write, grav, st);
+        launch_far_split<2>(a, precision, write, grav, q, n_cu, st);
     else
-        launch_far_split<1>(a, precision, write, grav, st);
+        launch_far_split<1>(a, precision, write, grav, q, n_cu, st);
 }
 
 TileArgs tile_args(FastBuffers& fb)
@@ -188,6 +217,7 @@ TileArgs tile_args(FastBuffers& fb)
     a.part = fb.part;
     a.nbm = fb.nbm;
     a.pe_tile = fb.pe_tile;
+    a.queue = fb.queue;
     return a;
 }
 
@@ -200,16 +230,21 @@ bool split_ready(const FastBuffers& fb) { return fb.split != 0 && fb.split_key =
 
 // Autotuner: the number of wavefronts sharing one far tile trades per-wave set-up
 // against occupancy and load balance, which depend on the systems' sizes and the
-// machine.  A choice stored for this configuration (sl_tune.cpp) is used as it is;
+// machine, and the work-queue launch trades a per-tile atomic for the launch's ramp and
+// tail.  A choice stored for this configuration (sl_tune.cpp) is used as it is;
 // otherwise each choice is warmed up once and then timed over three rounds of five
 // launches on the live state (HIP events on the engine's stream), and the one kept is
 // stored.  The trial launches only write partial sums that the real launch overwrites.
+// Returns the split (1, 2, 4) plus 8 for the queue; a forced split takes the grid launch.
 int tune_split(FastBuffers& fb, const TileArgs& a, hipStream_t st)
 {
     if (fb.forced_split == 1 || fb.forced_split == 2 || fb.forced_split == 4) return fb.forced_split;
     const std::string key = tune_key(fb.precision, gravity_on(fb) ? 1 : 0, fb.lay.n_sys, fb.lay.n_used,
                                      fb.lay.nb_max, fb.n_tiles);
-    if (const int cached = tune_cache_get(key)) return cached;
+    if (const int cached = tune_cache_get(key)) {
+        const int sp = cached & 7;
+        if (sp == 1 || sp == 2 || sp == 4) return cached & 15;
+    }
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return 1;
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -217,14 +252,16 @@ int tune_split(FastBuffers& fb, const TileArgs& a, hipStream_t st)
         return 1;
     }
     const bool grav = gravity_on(fb);
-    const int choices[3] = {1, 2, 4};
-    float t_min[3] = {1e30f, 1e30f, 1e30f};   // best of three rounds per choice
+    constexpr int NC = 6;
+    const int choices[NC] = {1, 2, 4, 9, 10, 12};
+    float t_min[NC];                            // best of three rounds per choice
+    for (int k = 0; k < NC; k++) t_min[k] = 1e30f;
     for (int round = 0; round < 3; round++)
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < NC; k++) {
             const int c = choices[k];
-            launch_far_any(a, c, fb.precision, false, grav, st);
+            launch_far_any(a, c, fb.precision, false, grav, fb.n_cu, st);
             (void)hipEventRecord(e0, st);
-            for (int rep = 0; rep < 5; rep++) launch_far_any(a, c, fb.precision, false, grav, st);
+            for (int rep = 0; rep < 5; rep++) launch_far_any(a, c, fb.precision, false, grav, fb.n_cu, st);
             (void)hipEventRecord(e1, st);
             if (hipEventSynchronize(e1) != hipSuccess) break;
             float ms = 0;
@@ -234,16 +271,21 @@ int tune_split(FastBuffers& fb, const TileArgs& a, hipStream_t st)
     (void)hipStreamSynchronize(st);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    // the smallest split within 2% of the fastest: splits that time alike then give the
-    // same choice (and the same sums) from run to run more often
-    const float fastest = std::min(t_min[0], std::min(t_min[1], t_min[2]));
-    int pick = 1;
+    // The split: the smallest within 2% of the fastest (splits that time alike then give
+    // the same choice, and the same sums, from run to run more often).  The launch: the
+    // queue only where it beats the grid launch at that split by more than 2% (both give
+    // the same bits).
+    float best_of_split[3];
+    for (int k = 0; k < 3; k++) best_of_split[k] = std::min(t_min[k], t_min[k + 3]);
+    const float fastest = std::min(best_of_split[0], std::min(best_of_split[1], best_of_split[2]));
+    int ks = 0;
     for (int k = 0; k < 3; k++)
-        if (t_min[k] <= 1.02f * fastest) {
-            pick = choices[k];
+        if (best_of_split[k] <= 1.02f * fastest) {
+            ks = k;
             break;
         }
-    tune_cache_put(key, pick, t_min, 3);
+    const int pick = t_min[ks + 3] < t_min[ks] / 1.02f ? choices[ks + 3] : choices[ks];
+    tune_cache_put(key, pick, t_min, NC);
     return pick;
 }
 
@@ -251,12 +293,14 @@ int launch_tiles_step(FastBuffers& fb, bool write, hipStream_t st, std::string* 
 {
     const TileArgs a = tile_args(fb);
     if (!split_ready(fb)) {
-        fb.split = tune_split(fb, a, st);
+        const int choice = tune_split(fb, a, st);
+        fb.split = choice & 7;
+        fb.queue_launch = fb.forced_queue ? fb.forced_queue == 2 : (choice & 8) != 0;
         fb.split_key = split_key(fb);
         fast_clear_graphs(fb);
     }
     if (fb.hook.before) fb.hook.before(fb.hook.user, 0);
-    launch_far_any(a, fb.split, fb.precision, write, gravity_on(fb), st);
+    launch_far_any(a, fb.split | (fb.queue_launch ? 8 : 0), fb.precision, write, gravity_on(fb), fb.n_cu, st);
     if (fb.hook.after) fb.hook.after(fb.hook.user, 0);
     if (hipGetLastError() != hipSuccess) return err_hip(err, "tile kernel launch");
     return 0;
@@ -525,6 +569,8 @@ int fast_alloc(FastBuffers& fb, int n_pad, int max_sys)
     if (hipMalloc(&fb.bnd, sizeof(double4) * 2 * nblocks) != hipSuccess) return -1;
     if (hipMalloc(&fb.xl, sizeof(float4) * n_pad) != hipSuccess) return -1;
     if (hipMalloc(&fb.gq_blk, sizeof(float) * nblocks) != hipSuccess) return -1;
+    if (hipMalloc(&fb.queue, sizeof(int) * QPAD * (QSEG + 1)) != hipSuccess) return -1;
+    if (hipMemset(fb.queue, 0, sizeof(int) * QPAD * (QSEG + 1)) != hipSuccess) return -1;
     if (hipMalloc(&fb.blk_sys, sizeof(int) * nblocks) != hipSuccess) return -1;
     if (hipMalloc(&fb.sys_off, sizeof(int) * max_sys) != hipSuccess) return -1;
     if (hipMalloc(&fb.sys_n, sizeof(int) * max_sys) != hipSuccess) return -1;
@@ -554,7 +600,7 @@ void fast_clear_graphs(FastBuffers& fb)
 void fast_free(FastBuffers& fb)
 {
     fast_clear_graphs(fb);
-    void* ptrs[] = {fb.eng_base, fb.perm, fb.aux.q, fb.aux.g, fb.aux.q32, fb.aux.g32, fb.bnd, fb.xl, fb.gq_blk, fb.blk_sys,
+    void* ptrs[] = {fb.eng_base, fb.perm, fb.aux.q, fb.aux.g, fb.aux.q32, fb.aux.g32, fb.bnd, fb.xl, fb.gq_blk, fb.queue, fb.blk_sys,
                     fb.sys_off, fb.sys_n, fb.sys_nb, fb.sysP, fb.tiles, fb.part, fb.nbm, fb.flags,
                     fb.blk_ctr, fb.e_blk, fb.pe_tile};
     for (void* p : ptrs)

@@ -120,6 +120,8 @@ def test_binding_constants_match_header():
     assert defs["SL_OPT_GRAPHS"] == sl.OPT_GRAPHS
     assert defs["SL_OPT_TILE_SPLIT"] == sl.OPT_TILE_SPLIT
     assert defs["SL_OPT_BALL_WAVES"] == sl.OPT_BALL_WAVES
+    assert defs["SL_OPT_ORDERED_SHAPE"] == sl.OPT_ORDERED_SHAPE
+    assert defs["SL_OPT_TILE_QUEUE"] == sl.OPT_TILE_QUEUE
     for code, name in sl.ERRORS.items():
         assert defs[name] == code, name
     assert (defs["SL_OUTCOME_STICK"], defs["SL_OUTCOME_BOUNCE"], defs["SL_OUTCOME_FRAGMENT"]) == (
@@ -129,8 +131,8 @@ def test_binding_constants_match_header():
 
 def test_shipped_tune_cache_is_valid():
     """The tune cache shipped beside the library: a flat JSON object whose entries name a
-    tile split (1, 2 or 4) or an ordered launch shape (1 + index, at most 64), with the
-    timings they were chosen from."""
+    tile split (1, 2 or 4, plus 8 for the work-queue launch) or an ordered launch shape
+    (1 + index, at most 64), with the timings they were chosen from."""
     import json
     path = os.path.join(os.path.dirname(sl.__file__), "data", "tune.json")
     data = json.load(open(path))
@@ -139,8 +141,8 @@ def test_shipped_tune_cache_is_valid():
         kind = key.split(":")[0]
         assert kind in ("tiles", "ordered"), key
         if kind == "tiles":
-            assert v["split"] in (1, 2, 4), key
-            assert len(v["ms"]) == 3
+            assert v["split"] in (1, 2, 4, 9, 10, 12), key
+            assert len(v["ms"]) in (3, 6)
         else:
             assert 1 <= v["split"] <= 64, key
             assert len(v["ms"]) >= 2

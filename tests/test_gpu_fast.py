@@ -428,5 +428,35 @@ def test_default_runs_repeat_bitwise_through_tune_cache(tmp_path):
         outs.append(r.stdout.split())
     assert outs[0] == outs[1]
     cache = json.loads((tmp_path / "tune.json").read_text())
-    tiles = [v["split"] for k, v in cache.items() if k.startswith("tiles:prec0:")]
+    # the stored choice: the split, plus 8 when the tiles come from the work queue
+    tiles = [v["split"] & 7 for k, v in cache.items() if k.startswith("tiles:prec0:")]
     assert tiles == [int(outs[0][0])]
+
+
+@pytest.mark.parametrize("prec", [sl.PREC_FP64, sl.PREC_MIXED])
+def test_tile_queue_launch_same_bits(prec):
+    """The work-queue launch of the tile kernel (as many workgroups as the device holds,
+    pulling tiles) gives the bits of the one-workgroup-per-tile launch, at every split,
+    over steps with and without energies; and two queue runs in a row (the queue rewinds
+    itself after each launch) agree too."""
+    n = 1500
+    g = sl.generate_bpca(n, seed=21, v_sigma=3.0)
+    p = sl.calibrate(sl.preset("config"), g.R, g.m, velocity=V_CUSTOM, cor=0.64).replace(
+        order=sl.ORDER_FAST, precision=prec)
+    for split in (1, 2, 4):
+        outs = []
+        for queue in (1, 2, 2):
+            with sl.Collider(n, p) as c:
+                c.set_option("tile_split", split)
+                c.set_option("tile_queue", queue)
+                c.upload(g)
+                e0 = c.init_conditions()
+                ew = c.step(40, 10)
+                assert c.get_option("tile_queue") == queue
+                st = c.download()
+            outs.append((st, [e0] + ew))
+        for st, ee in outs[1:]:
+            for k in ("pos", "vel", "w", "acc", "aacc"):
+                assert bits_equal(getattr(st, k), getattr(outs[0][0], k)), (split, k)
+            for a, b in zip(ee, outs[0][1]):
+                assert a.PE == b.PE and a.KE == b.KE and a.contacts == b.contacts
