@@ -1,7 +1,7 @@
 // Surrogate model for collision outcomes: a 4-layer perceptron
 //   x (n_in) -> 256 -> 256 -> 256 -> y (n_out), ReLU between layers,
-// whose matrix products run on the matrix cores (v_mfma_f32_16x16x16bf16_1k: bfloat16
-// operands, binary32 accumulation).  Training (forward, loss, backward, Adam) and
+// whose matrix products run on the matrix cores (gfx950's v_mfma_f32_16x16x32_bf16:
+// bfloat16 operands, binary32 accumulation, 32 of K per instruction).  Training (forward, loss, backward, Adam) and
 // inference are all device kernels; every reduction has a fixed order, so a run is
 // reproducible bit for bit.
 //
@@ -22,7 +22,7 @@
 namespace {
 
 using bf16 = unsigned short;
-typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 constexpr int NL = 4;          // layers
@@ -52,14 +52,22 @@ __host__ __device__ inline float from_bf16(bf16 h)
 // accumulation.  op(A)(m, k) = TA ? A[k * lda + m] : A[m * lda + k]; op(B)(k, n) =
 // TB ? B[n * ldb + k] : B[k * ldb + n].  Optional ReLU; the result is stored in binary32
 // (C) and/or bfloat16 (Cb).  A 256-thread block computes a 32 x 32 tile: wavefront w
-// owns the 16 x 16 quarter (w / 2, w % 2), one v_mfma_f32_16x16x16bf16_1k per 16 of K.
-// Operand tiles of GK = 64 along K are staged through LDS (zero fill outside the
-// matrices); global reads run along each matrix's contiguous index, and both tiles are
-// stored K-contiguous so each lane reads its four K values as one 8-byte word.  The
-// small output tile keeps many workgroups in flight for the 256-wide layers.
+// owns the 16 x 16 quarter (w / 2, w % 2), one v_mfma_f32_16x16x32_bf16 per 32 of K
+// (lane l: row or column l % 16, K values 8 (l / 16) .. 8 (l / 16) + 7 of the step, the
+// same K values for both operands).  Operand tiles of GK = 64 along K are staged through
+// LDS (zero fill outside the matrices); global reads run along each matrix's contiguous
+// index, and both tiles are stored K-contiguous so each lane reads its eight K values as
+// one 16-byte word.  The small output tile keeps many workgroups in flight for the
+// 256-wide layers.
+//
+// With csum, the blocks of the first row of tiles also form the column sums of the
+// binary32 matrix csrc[K][ldcs] over its K rows for their GN columns (the bias gradient
+// beside the weight gradient X^T dZ): 8 row phases per column, phase p adding rows p,
+// p + 8, ... in order, then the phases added in order.
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16* A, int lda, const bf16* B, int ldb,
-                                              float* C, int ldc, bf16* Cb, int ldcb, const float* bias, int relu)
+                                              float* C, int ldc, bf16* Cb, int ldcb, const float* bias, int relu,
+                                              const float* csrc, int ldcs, float* csum)
 {
     __shared__ __attribute__((aligned(16))) bf16 As[GM][GK + 8];   // [m][k]
     __shared__ __attribute__((aligned(16))) bf16 Bt[GN][GK + 8];   // [n][k]
@@ -99,13 +107,31 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16* A
         __syncthreads();
         const int row = wm + (l % 16), col = wn + (l % 16);
 #pragma unroll
-        for (int kk = 0; kk < GK; kk += 16) {
-            const int kq = kk + 4 * (l / 16);
-            const short4_t a = *reinterpret_cast<const short4_t*>(&As[row][kq]);
-            const short4_t b = *reinterpret_cast<const short4_t*>(&Bt[col][kq]);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        for (int kk = 0; kk < GK; kk += 32) {
+            const int kq = kk + 8 * (l / 16);
+            const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&As[row][kq]);
+            const bf16x8_t b = *reinterpret_cast<const bf16x8_t*>(&Bt[col][kq]);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
         }
         __syncthreads();
+    }
+    if (csum && blockIdx.y == 0) {
+        __shared__ float part[8][GN];
+        const int cl = tid % GN, ph = tid / GN;   // 8 phases of GN columns
+        const int c = n0 + cl;
+        float sum = 0.f;
+        if (c < N) {
+            // the loads run ahead of the (ordered) additions
+#pragma unroll 8
+            for (int r = ph; r < K; r += 8) sum += csrc[(size_t)r * ldcs + c];
+        }
+        part[ph][cl] = sum;
+        __syncthreads();
+        if (ph == 0 && c < N) {
+            float v = part[0][cl];
+            for (int q = 1; q < 8; q++) v += part[q][cl];
+            csum[c] = v;
+        }
     }
     for (int r = 0; r < 4; r++) {
         const int row = m0 + wm + 4 * (l / 16) + r;
@@ -173,22 +199,6 @@ __global__ void k_relu_back(int n, const float* z, float* dz, bf16* dzb)
     const float g = z[i] > 0.f ? dz[i] : 0.f;
     dz[i] = g;
     dzb[i] = to_bf16(g);
-}
-
-// Column sums of dz[B][ld] for the first n columns (bias gradient) in a fixed order:
-// 64 columns per 256-thread block, row phase p = 0..3 of each column summing rows
-// p, p + 4, ... in order, then the four phases added in order.
-__global__ __launch_bounds__(256) void k_colsum(int B, int n, int ld, const float* dz, float* db)
-{
-    __shared__ float part[4][64];
-    const int cl = threadIdx.x % 64, ph = threadIdx.x / 64;
-    const int c = blockIdx.x * 64 + cl;
-    float s = 0.f;
-    if (c < n)
-        for (int r = ph; r < B; r += 4) s += dz[(size_t)r * ld + c];
-    part[ph][cl] = s;
-    __syncthreads();
-    if (ph == 0 && c < n) db[c] = ((part[0][cl] + part[1][cl]) + part[2][cl]) + part[3][cl];
 }
 
 // Adam (Kingma & Ba 2015) on binary32 master weights, refreshing the bfloat16 copy.
@@ -269,10 +279,12 @@ int mlp_err(sl_mlp* m, int code, const char* what)
 
 template <bool TA, bool TB>
 void gemm(hipStream_t st, int M, int N, int K, const bf16* A, int lda, const bf16* B, int ldb, float* C, int ldc,
-          bf16* Cb, int ldcb, const float* bias, bool relu)
+          bf16* Cb, int ldcb, const float* bias, bool relu, const float* csrc = nullptr, int ldcs = 0,
+          float* csum = nullptr)
 {
     const dim3 grid((N + GN - 1) / GN, (M + GM - 1) / GM);
-    k_gemm<TA, TB><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, Cb, ldcb, bias, relu ? 1 : 0);
+    k_gemm<TA, TB><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, Cb, ldcb, bias, relu ? 1 : 0, csrc, ldcs,
+                                         csum);
 }
 
 void free_mlp(sl_mlp* m)
@@ -315,9 +327,9 @@ void backward(sl_mlp* m, int B)
     for (int k = NL - 1; k >= 0; k--) {
         Layer& L = m->L[k];
         const bf16* Hprev = k ? m->Hb[k - 1] : m->xb;
+        // weight gradient H^T dZ, and the bias gradient (column sums of dZ) in the same launch
         gemm<true, false>(m->stream, L.ldin, L.ldout, B, Hprev, L.ldin, m->dZb[k], L.ldout, L.gW, L.ldout, nullptr, 0,
-                          nullptr, false);
-        k_colsum<<<(L.ldout + 63) / 64, 256, 0, m->stream>>>(B, L.ldout, L.ldout, m->dZ[k], L.gb);
+                          nullptr, false, m->dZ[k], L.ldout, L.gb);
         if (k > 0) {
             gemm<false, true>(m->stream, B, L.ldin, L.ldout, m->dZb[k], L.ldout, L.Wb, L.ldout, m->dZ[k - 1], L.ldin,
                               nullptr, 0, nullptr, false);
