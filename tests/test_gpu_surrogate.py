@@ -64,6 +64,6 @@ def test_cascade_gain_and_agreement():
     assert r["agreement"] >= 0.95
     assert r["one_gate"]["agreement"] >= 0.95
     assert r["gain_streaming"] > r["one_gate"]["gain_streaming"]
-    # measured 10.98x on MI355X; the bar leaves room for the timing of the 512 DEM reruns
+    # measured 10.85-10.98x on MI355X; the bar leaves room for the timing of the 512 DEM reruns
     assert r["gain_streaming"] >= 10.0
     assert r["raw_accuracy"] >= 0.8
