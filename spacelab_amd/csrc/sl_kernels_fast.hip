@@ -673,20 +673,25 @@ int fast_configure(FastBuffers& fb, State& canon, const FastLayout& lay, const P
         std::vector<int4> tiles;
         fb.tile_sys_h.clear();
         // The diagonal tiles of every system first (half a sweep each, and most of the
-        // per-pair near tests), then the others row by row: measured 0.7% faster than row
-        // order for one aggregate and for a 512-collision batch.  Each system's tiles keep
-        // the same relative order whether it runs alone or in a batch.
-        for (int pass = 0; pass < 2; pass++)
+        // per-pair near tests), then the others by band, I - J = 1, 2, ...: blocks close in
+        // the space-filling-curve order are close in space, so the tiles that need the
+        // per-pair near test come early and the launch ends on far tiles.  (Diagonal first
+        // was measured 0.7% faster than row order.)  The order only schedules the tiles;
+        // each tile's sums and the slots they land in do not depend on it.  Each system's
+        // tiles keep the same relative order whether it runs alone or in a batch.
+        for (int s = 0; s < lay.n_sys; s++) {
+            const int b0 = lay.off[s] / TB;
+            for (int k = 0; k < lay.nb[s]; k++) fb.blk_sys_h[b0 + k] = s;
+        }
+        int nb_top = 0;
+        for (int s = 0; s < lay.n_sys; s++) nb_top = std::max(nb_top, lay.nb[s]);
+        for (int d = 0; d < nb_top; d++)
             for (int s = 0; s < lay.n_sys; s++) {
                 const int b0 = lay.off[s] / TB;
-                if (pass == 0)
-                    for (int k = 0; k < lay.nb[s]; k++) fb.blk_sys_h[b0 + k] = s;
-                for (int I = 0; I < lay.nb[s]; I++)
-                    for (int J = 0; J <= I; J++)
-                        if ((I == J) == (pass == 0)) {
-                            tiles.push_back(make_int4(b0 + I, b0 + J, s, b0));
-                            fb.tile_sys_h.push_back(s);
-                        }
+                for (int I = d; I < lay.nb[s]; I++) {
+                    tiles.push_back(make_int4(b0 + I, b0 + I - d, s, b0));
+                    fb.tile_sys_h.push_back(s);
+                }
             }
         const int nt = (int)tiles.size();
         if (nt > fb.cap_tiles) {
