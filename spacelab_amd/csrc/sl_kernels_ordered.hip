@@ -467,7 +467,7 @@ struct OrdShape {
 };
 constexpr OrdShape ORD_SHAPES[] = {
     {4, 4, 128, 0}, {4, 2, 64, 0}, {4, 4, 64, 0}, {8, 4, 32, 0}, {2, 2, 128, 0},
-    {8, 4, 64, 0},  {2, 2, 128, 0}, {8, 4, 64, 0}, {4, 2, 64, 0},
+    {8, 4, 64, 0},  {2, 2, 128, 0}, {2, 4, 128, 0}, {2, 4, 128, 0},
 };
 constexpr int N_ORD_SHAPES = (int)(sizeof(ORD_SHAPES) / sizeof(ORD_SHAPES[0]));
 }  // namespace
@@ -475,10 +475,10 @@ constexpr int N_ORD_SHAPES = (int)(sizeof(ORD_SHAPES) / sizeof(ORD_SHAPES[0]));
 int ordered_shapes() { return N_ORD_SHAPES; }
 
 // The shape used when none is chosen (the allocation-free entry point, and until the
-// tuner has run), from timings on MI355X: up to about 1500 balls 4 balls per workgroup
-// and chunks of 128 partners; above, chunks of 64 and registers held to 3 wavefronts per
-// SIMD.
-int ordered_default_shape(int n) { return n < 1500 ? 0 : 2; }
+// tuner has run), from timings on MI355X: up to about 1500 balls 2 balls per workgroup of
+// four wavefronts and chunks of 128 partners; above, 4 balls, chunks of 64 and registers
+// held to 3 wavefronts per SIMD.
+int ordered_default_shape(int n) { return n < 1500 ? 8 : 2; }
 
 int ordered_blocks(int n, int shape)
 {
@@ -502,8 +502,8 @@ hipError_t launch_pairs_ordered(const State& s, const Phys& P, int n, int write,
     case 4: k_pairs_ordered<2, 2, 128, false, false, 0><<<nb, 128, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
     case 5: k_pairs_ordered<8, 4, 64, false, false, 0><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
     case 6: k_pairs_ordered<2, 2, 128, false, true, 0><<<nb, 128, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
-    case 7: k_pairs_ordered<8, 4, 64, false, false, 0, 2><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
-    default: k_pairs_ordered<4, 2, 64, false, false, 0, 2><<<nb, 128, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
+    case 7: k_pairs_ordered<2, 4, 128, false, false, 3><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
+    default: k_pairs_ordered<2, 4, 128, false, false, 0><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
     }
     return hipGetLastError();
 }
