@@ -453,10 +453,11 @@ def surrogate_eval():
             "raw_accuracy", "stage1_share", "dem_share", "gate_target", "look_ahead_steps", "n_train", "n_val",
             "n_heldout", "n_models", "epochs", "train_s", "dem_s", "cascade_s", "surrogate_ms_per_collision",
             "dem_ms_per_collision", "reg_mse", "reg_mse_mean_predictor", "dem_steps", "softening", "one_gate",
-            "dem_labels_match_stored")
+            "dem_labels_match_stored", "batch_systems")
     out = {k: r[k] for k in keys}
-    out["gain_note"] = ("gain: DEM-alone wall time over the cascade's on the same 10,000 collisions (fallbacks "
-                        "run in partly filled batches); gain_streaming: per collision, full-batch DEM time over "
+    out["gain_note"] = ("gain: DEM-alone wall time over the cascade's on the same 10,000 collisions (both with up "
+                        "to batch_systems collisions per batched launch); gain_streaming: per collision, full-batch "
+                        "DEM time over "
                         "the surrogate's own time plus the look-ahead share at 4000/40000 of a DEM collision and "
                         "the run-to-the-end share at one")
     out["scale"] = ("targets 32-512 balls, projectiles 1-64, 10-1000 cm/s, b uniform; outcome after "

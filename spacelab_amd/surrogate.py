@@ -27,7 +27,7 @@ import argparse
 import json
 import math
 import time
-from dataclasses import asdict, dataclass, field
+from dataclasses import asdict, dataclass, field, replace
 
 import numpy as np
 
@@ -460,7 +460,7 @@ def merge_datasets(parts: list[dict]) -> dict:
 
 def evaluate_cascade(train: dict, heldout: dict, target: float = 0.95, z: float = 2.0, val_frac: float = 0.5,
                      epochs: int = 20, lr: float = 1e-3, n_models: int = 4, seed: int = 1,
-                     dem_all: bool = True, check: int = 512, log=None) -> dict:
+                     dem_all: bool = True, check: int = 512, batch_systems: int = 1024, log=None) -> dict:
     """The cascade and the one-gate surrogate on the held-out collisions.
 
     Both stages train on ``train`` with a validation share held back; the thresholds of
@@ -475,6 +475,11 @@ def evaluate_cascade(train: dict, heldout: dict, target: float = 0.95, z: float 
     Twenty epochs, not forty: the shorter training leaves the networks less sure of
     themselves where they are wrong, and both gates' validation cost fell (DEM per
     collision 0.0986 -> 0.0888 for the cascade, fallback share 10.2% -> 9.4% for one gate).
+
+    DEM runs (alone, look-ahead and to the end) take up to ``batch_systems`` collisions per
+    batched launch; DEM alone keeps the GPU full either way, while the cascade's fallbacks
+    (a few hundred collisions) then run as one batch instead of a full one and a partial
+    one.  At a fixed tile split a collision's result does not depend on its batch.
 
     The streaming gain counts per collision the surrogate's own time (descriptors and
     both stages), the look-ahead share at look/steps of a full-batch DEM collision and
@@ -503,9 +508,10 @@ def evaluate_cascade(train: dict, heldout: dict, target: float = 0.95, z: float 
             log(f"trained two stages in {t_train:.1f} s; one gate {gate1}; cascade {gate2}")
         specs = specs_of(heldout)
         preps = [prepare(q, s) for q in specs]
-        max_balls = s.max_systems * max(q.setup.group.num_particles for q in preps)
-        dem = DemRunner(s, max_balls)
-        dem.run(preps[:16], steps=256)          # first-use set-up (tile-split tuning, graphs)
+        s_run = replace(s, max_systems=batch_systems)
+        max_balls = s_run.max_systems * max(q.setup.group.num_particles for q in preps)
+        dem = DemRunner(s_run, max_balls)
+        dem.run(preps[:16], steps=256)          # first-use set-up (graphs)
         ref = heldout["cls"].astype(np.int32)
         if dem_all:
             t0 = time.perf_counter()
@@ -520,7 +526,7 @@ def evaluate_cascade(train: dict, heldout: dict, target: float = 0.95, z: float 
         stored_match = float((c_dem == ref[:len(c_dem)]).mean())
         if dem_all:
             ref = c_dem
-        c_c, r_c, m1, m2, Lh, tc = run_cascade(preps, cas, dem, s)
+        c_c, r_c, m1, m2, Lh, tc = run_cascade(preps, cas, dem, s_run)
         Xh = descriptors(preps, s)
         p1, _, _, e1 = cas.s1.predict(Xh)
         look_diff = float(np.abs(Lh - heldout[look_key][m1]).max()) if m1.any() else 0.0
@@ -539,7 +545,7 @@ def evaluate_cascade(train: dict, heldout: dict, target: float = 0.95, z: float 
     out = {
         "n_train": int(len(tr)), "n_val": int(nv), "n_heldout": nh, "epochs": epochs, "n_models": n_models,
         "train_s": t_train, "gate_target": target + margin, "look_ahead_steps": LOOK_AHEAD,
-        "dem_steps": s.steps, "softening": s.softening,
+        "dem_steps": s.steps, "softening": s.softening, "batch_systems": batch_systems,
         "raw_accuracy": float((p1 == ref).mean()),
         "agreement": float((c_c == ref).mean()), "stage1_share": f1, "dem_share": f2,
         "gain_streaming": c_d / (c_s + f1 * look_cost * c_d + f2 * c_d),
