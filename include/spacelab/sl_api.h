@@ -121,10 +121,13 @@ long long sl_step_count(const sl_ctx* ctx);
 const char* sl_last_error(const sl_ctx* ctx);
 void sl_destroy(sl_ctx* ctx);
 
-/* Options of the SL_ORDER_FAST engine (no reference counterpart).
+/* Options (no reference counterpart).
  *   SL_OPT_GRAPHS      1 (default): runs of steps between write steps are captured
- *                      once as HIP graphs and replayed; 0: every kernel launched
- *                      directly.  Results are identical either way.
+ *                      once as HIP graphs and replayed, in both orders (not while
+ *                      sl_profile_enable times the launches, nor under a profiling
+ *                      tool); 0: every kernel launched directly.  Results are identical
+ *                      either way.
+ * The others are options of the SL_ORDER_FAST engine:
  *   SL_OPT_TILE_SPLIT  wavefronts per 64x64 pair tile: 0 (default) = chosen by the
  *                      built-in autotuner on first use of a configuration and stored in
  *                      a JSON cache ($SPACELAB_TUNE_CACHE, else ~/.cache/spacelab_amd/

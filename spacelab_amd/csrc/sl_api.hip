@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <new>
 #include <string>
 #include <vector>
@@ -32,6 +33,7 @@ struct sl_ctx {
     unsigned long long* cnt = nullptr;  // [0] contacts total, [1] contacts this write step, [2..3] tuner trials
     int ord_shape = -1;         // launch shape of the ordered pair pass (-1: not chosen yet)
     int forced_ord_shape = 0;   // 1 + shape forces it
+    std::map<int, hipGraphExec_t> ord_graphs;   // captured runs of ordered non-write steps, by length
     int* nonfinite = nullptr;
     double* h_pe = nullptr;     // pinned
     double* h_e = nullptr;      // pinned
@@ -226,6 +228,54 @@ int read_energy(sl_ctx* c, int n_pe_blocks, int n_e_blocks, sl_energy* out, bool
 }
 
 // One ordered pair pass and per-ball pass for the current positions.
+// Captured runs of ordered steps hold the parameters, the launch shape and the state's
+// size in their kernel arguments: they are dropped (on an idle stream) whenever one of
+// those changes.
+void clear_ord_graphs(sl_ctx* c)
+{
+    if (c->ord_graphs.empty()) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& kv : c->ord_graphs)
+        if (kv.second) (void)hipGraphExecDestroy(kv.second);
+    c->ord_graphs.clear();
+}
+
+constexpr int ORD_GRAPH_MAX_STEPS = 128;
+
+// len ordered steps without energies (drift, pair pass, closing kick each), as one
+// replay of a graph captured on first use: the same kernels with the same arguments as
+// the direct launches, so the same bits.
+int ordered_run(sl_ctx* c, int len)
+{
+    hipGraphExec_t ex = nullptr;
+    auto it = c->ord_graphs.find(len);
+    if (it != c->ord_graphs.end()) {
+        ex = it->second;
+    } else {
+        const Phys P = phys_of(c->p);
+        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return set_err(c, SL_E_HIP, "ordered step capture");
+        bool ok = true;
+        for (int k = 0; k < len && ok; k++)
+            ok = launch_drift(c->st, c->n, c->p.dt, c->stream) == hipSuccess &&
+                 launch_pairs_ordered(c->st, P, c->n, 0, c->pe_part, c->cnt, c->stream, c->ord_shape) == hipSuccess &&
+                 launch_kick(c->st, c->n, c->p.dt, 0, 1, c->e_part, c->nonfinite, c->stream) == hipSuccess;
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (!ok || e != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            return set_err(c, SL_E_HIP, "ordered step capture");
+        }
+        const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess || !ex) return set_err(c, SL_E_HIP, "ordered step graph instantiation");
+        c->ord_graphs[len] = ex;
+    }
+    HIPCHK(c, hipGraphLaunch(ex, c->stream));
+    return SL_OK;
+}
+
 int pair_pass(sl_ctx* c, bool write, bool kick, int* n_pe_blocks)
 {
     const Phys P = phys_of(c->p);
@@ -392,6 +442,7 @@ int sl_set_params(sl_ctx* c, const sl_params* p)
         if (rc) return rc;
     }
     if (p->far_gravity != c->p.far_gravity) c->ord_shape = -1;
+    clear_ord_graphs(c);
     c->p = *p;
     return SL_OK;
 }
@@ -403,6 +454,7 @@ int sl_set_option(sl_ctx* c, int option, int value)
     case SL_OPT_GRAPHS:
         c->fb.use_graphs = value != 0;
         fast_clear_graphs(c->fb);
+        clear_ord_graphs(c);
         return SL_OK;
     case SL_OPT_TILE_SPLIT:
         if (value != 0 && value != 1 && value != 2 && value != 4)
@@ -429,6 +481,7 @@ int sl_set_option(sl_ctx* c, int option, int value)
             return set_err(c, SL_E_ARG, "ordered shape must be 0 (tuned) or 1..%d (got %d)", ordered_shapes(), value);
         c->forced_ord_shape = value;
         c->ord_shape = -1;
+        clear_ord_graphs(c);
         return SL_OK;
     default:
         return set_err(c, SL_E_ARG, "unknown option %d", option);
@@ -505,6 +558,7 @@ int sl_upload(sl_ctx* c, int n, const double* pos3, const double* vel3, const do
     HIPCHK(c, hipMemcpyAsync(c->base, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (n != c->n) c->ord_shape = -1;
+    clear_ord_graphs(c);
     c->n = n;
     c->step = 0;
     const int rc = configure_fast(c);
@@ -552,8 +606,28 @@ int sl_step(sl_ctx* c, int n_steps, int write_every, sl_energy* e_out, int e_cap
         return nw;
     }
     if (n_steps > 0) fast_canon_changed(c->fb);
+    // Runs of steps without energies replay captured graphs (not while profiling, where
+    // each launch is timed, nor under a profiling tool); write steps, and the step that
+    // chooses the launch shape, are launched directly.
+    const bool graphs = c->fb.use_graphs && !c->prof && !tool_attached();
     int nw = 0;
-    for (int k = 0; k < n_steps; k++) {
+    for (int k = 0; k < n_steps;) {
+        if (graphs && c->ord_shape >= 0) {
+            int len = 0;
+            while (k + len < n_steps && len < ORD_GRAPH_MAX_STEPS) {
+                const long long s = c->step + 1 + len;
+                if (write_every > 0 && s % write_every == 0) break;
+                len++;
+            }
+            if (len > 0) {
+                const int rc = ordered_run(c, len);
+                if (rc) return rc;
+                c->step += len;
+                k += len;
+                continue;
+            }
+        }
+        k++;
         c->step++;
         const bool write = write_every > 0 && (c->step % write_every) == 0;
         HIPCHK(c, launch_drift(c->st, c->n, c->p.dt, c->stream));
@@ -637,6 +711,7 @@ void sl_destroy(sl_ctx* c)
 {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
+    clear_ord_graphs(c);
     for (auto e : c->ev_pool) hipEventDestroy(e);
     free_ctx_buffers(c);
     if (c->stream) hipStreamDestroy(c->stream);

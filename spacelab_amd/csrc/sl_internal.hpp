@@ -148,6 +148,10 @@ int fast_ball_waves(const FastBuffers& fb);
 hipError_t fast_reset_counters(FastBuffers& fb, hipStream_t st);
 int fast_read_counters(FastBuffers& fb, hipStream_t st, long long* contacts, long long* near_pairs);
 
+// A profiling tool (rocprofv3) is attached to this process: launch kernels directly, no
+// graph replay.
+bool tool_attached();
+
 // ---- the autotuner's persistent cache: sl_tune.cpp
 std::string tune_key(int precision, int gravity, int n_sys, int n_used, int nb_max, int n_tiles);
 int tune_cache_get(const std::string& key);   // the stored split, 0 if none

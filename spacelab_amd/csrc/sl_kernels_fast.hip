@@ -111,22 +111,6 @@ __global__ void k_probe_rot(int* out)
 
 constexpr int GRAPH_MAX_STEPS = 128;   // steps per captured graph
 
-// A profiling tool loaded through rocprofiler-sdk (rocprofv3) is named in
-// ROCP_TOOL_LIBRARIES of the process it profiles.  Runs under rocprofv3 --kernel-trace
-// that replayed captured step graphs ended three times in host-side SIGSEGVs inside the
-// runtime, below hipGraphLaunch, while the same runs without the tool pass; graph
-// executables here are only destroyed on an idle stream and re-captured after every
-// reconfiguration.  With a tool attached the engine therefore launches every kernel
-// directly (the kernels and their order are the same).
-bool tool_attached()
-{
-    static const bool on = [] {
-        const char* v = std::getenv("ROCP_TOOL_LIBRARIES");
-        return v != nullptr && *v != 0;
-    }();
-    return on;
-}
-
 int err_hip(std::string* err, const char* what)
 {
     if (err) *err = std::string("fast path: ") + what;
@@ -519,6 +503,22 @@ int ensure_engine(FastBuffers& fb, State& canon, hipStream_t st, std::string* er
 }
 
 }  // namespace
+
+// A profiling tool loaded through rocprofiler-sdk (rocprofv3) is named in
+// ROCP_TOOL_LIBRARIES of the process it profiles.  Runs under rocprofv3 --kernel-trace
+// that replayed captured step graphs ended three times in host-side SIGSEGVs inside the
+// runtime, below hipGraphLaunch, while the same runs without the tool pass; graph
+// executables here are only destroyed on an idle stream and re-captured after every
+// reconfiguration.  With a tool attached the engines therefore launch every kernel
+// directly (the kernels and their order are the same).
+bool tool_attached()
+{
+    static const bool on = [] {
+        const char* v = std::getenv("ROCP_TOOL_LIBRARIES");
+        return v != nullptr && *v != 0;
+    }();
+    return on;
+}
 
 // Four wavefronts (three workgroups per CU) when there are more 64-ball blocks than CUs
 // and every system has at most 80 blocks: many small systems, as in a collision

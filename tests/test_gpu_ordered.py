@@ -296,3 +296,28 @@ def test_ordered_shapes_bitwise(oracle_mod):
         for x, r in zip(ew, oew):
             _energy_close(x, r, g.m, g.moi, out.pos, out.vel, out.w, rtol=1e-12)
         assert ew[-1].contacts > 0
+
+
+def test_ordered_graphs_same_bits():
+    """Runs of ordered steps between write steps replay captured graphs; with graphs off
+    every kernel is launched directly.  Both give the same bits in every state array and
+    the same energies, over runs that end on and between write steps and across calls of
+    different lengths (so several captured lengths are replayed)."""
+    n = 300
+    g = sl.generate_bpca(n, seed=77, v_sigma=5.0)
+    p = sl.calibrate(sl.preset("config"), g.R, g.m, velocity=0.36301555459799423, cor=0.64)
+    outs = []
+    for graphs in (0, 1):
+        with sl.Collider(n, p) as c:
+            c.set_option("graphs", graphs)
+            c.upload(g)
+            c.init_conditions()
+            ew = c.step(130, 40) + c.step(75, 40) + c.step(3, 0) + c.step(200, 100)
+            out = c.download()
+        outs.append((out, ew))
+    for k in ("pos", "vel", "velh", "w", "wh", "acc", "aacc"):
+        assert bits_equal(getattr(outs[0][0], k), getattr(outs[1][0], k)), k
+    assert len(outs[0][1]) == len(outs[1][1]) == 7
+    for a, b in zip(outs[0][1], outs[1][1]):
+        assert a.PE == b.PE and a.KE == b.KE and a.contacts == b.contacts
+        assert np.array_equal(a.p, b.p) and np.array_equal(a.L, b.L)
