@@ -698,38 +698,52 @@ def main():
 
     cpu = None
     extra = {}
+
+    def leg(name, fn, done=None):
+        """One secondary measurement; its failure is recorded in the line rather than losing
+        the line."""
+        try:
+            out = fn()
+        except Exception as e:      # noqa: BLE001 -- reported, the headline is still printed
+            log(f"{name}: failed: {type(e).__name__}: {e}")
+            return {"error": f"{type(e).__name__}: {e}"}
+        if done:
+            log(done(out))
+        return out
+
     if not a.no_cpu:
         log("cpu baseline: reference sim_one_step on one host core (bounded sample)")
-        cpu = cpu_baseline(g, p, a.cpu_seconds)
+        cpu = leg("cpu baseline", lambda: cpu_baseline(g, p, a.cpu_seconds))
+        if "error" in cpu:
+            cpu = None
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle ordered gather on {threads} threads")
-        extra["cpu_omp"] = cpu_omp(g, p, threads)
+        extra["cpu_omp"] = leg("cpu omp", lambda: cpu_omp(g, p, threads))
 
     if not a.no_config0:
         threads = min(16, os.cpu_count() or 1)
         log(f"config 0: 1000 spheres, 10k steps on the GPU and on the host ({threads} threads; "
             f"reference single core: {not a.no_config0_single})")
-        extra["config0"] = config0(threads=threads, single_core=not a.no_config0_single)
+        extra["config0"] = leg("config 0", lambda: config0(threads=threads, single_core=not a.no_config0_single))
     if not a.no_e2e:
         log("end to end: spacelab_collider, 5000 spheres, 20000 steps, write every 1000")
-        extra["end_to_end"] = end_to_end()
-        log(f"end to end: {extra['end_to_end']['pairs_per_s']:.4e} pairs/s")
+        extra["end_to_end"] = leg("end to end", end_to_end, lambda o: f"end to end: {o['pairs_per_s']:.4e} pairs/s")
     if not a.no_collision:
         log("collision: 2000 + 200 spheres, 50000 steps, fp64 and mixed")
-        extra["collision"] = collision()
-        log(f"collision: fp64 {extra['collision']['fp64']['seconds']:.2f} s, "
-            f"mixed {extra['collision']['mixed']['seconds']:.2f} s, outcome {extra['collision']['fp64']['outcome']}")
+        extra["collision"] = leg("collision", collision,
+                                 lambda o: f"collision: fp64 {o['fp64']['seconds']:.2f} s, mixed "
+                                           f"{o['mixed']['seconds']:.2f} s, outcome {o['fp64']['outcome']}")
     if not a.no_ensemble:
         log(f"ensemble: 512 collisions at full size, {a.ensemble_steps} timed steps")
-        extra["ensemble"] = ensemble(steps=a.ensemble_steps)
-        log(f"ensemble: {extra['ensemble']['ms_per_step']:.3f} ms/step, "
-            f"{extra['ensemble']['collisions_per_s']:.3f} collisions/s at 50k steps")
+        extra["ensemble"] = leg("ensemble", lambda: ensemble(steps=a.ensemble_steps),
+                                lambda o: f"ensemble: {o['ms_per_step']:.3f} ms/step, "
+                                          f"{o['collisions_per_s']:.3f} collisions/s at 50k steps")
     if not a.no_surrogate:
         log("surrogate: training on the stored collisions, then DEM alone vs the look-ahead cascade on the "
             "10,000 held-out collisions")
-        extra["surrogate"] = surrogate_eval()
-        log(f"surrogate: gain {extra['surrogate']['gain']:.2f}x wall clock, "
-            f"{extra['surrogate']['gain_streaming']:.2f}x streaming, at agreement {extra['surrogate']['agreement']:.4f}")
+        extra["surrogate"] = leg("surrogate", surrogate_eval,
+                                 lambda o: f"surrogate: gain {o['gain']:.2f}x wall clock, "
+                                           f"{o['gain_streaming']:.2f}x streaming, at agreement {o['agreement']:.4f}")
 
     line = {"metric": "pair-interactions/s FP64 vs mixed at stated energy error; collisions/s surrogate",
             "value": r["pairs_per_s"], "unit": "pair-interactions/s", "n_gpus": 1,
