@@ -473,8 +473,9 @@ def surrogate_eval():
         "the stated workload (20k training and 2000 held-out collisions from the 512-collision ensemble's "
         "distribution: targets 200-4000 spheres, projectiles 1-500, springs calibrated at the impact speed) "
         "is not run: at dt = t_c/50 a collision advances about one median radius per 24,000 steps whatever "
-        "its speed, so finishing one takes about 5e5 steps, and labelling 22,000 at the ensemble's measured "
-        "1.65 ms per 512-collision step would take about 9 GPU-hours")
+        "its speed, so by the finish criterion (2 (R_t + R_p) of relative travel) the median collision of "
+        "that distribution needs 2.2 million steps; labelling 22,000 at the ensemble's measured 1.66 ms per "
+        "512-collision step would take about 45 GPU-hours with finished slots refilled")
     return out
 
 
