@@ -111,8 +111,10 @@ def _check(g, p, steps, write_every, hip):
 
 
 def test_static_entry_matches_sl_step_twice():
+    """2000 spheres, then 300: the two sizes take different default launch shapes of the
+    ordered pass (1500 spheres and above, and below)."""
     hip = _Hip()
-    g1 = sl.generate_bpca(700, seed=0x5EED0A01, v_sigma=2.0)
+    g1 = sl.generate_bpca(2000, seed=0x5EED0A01, v_sigma=2.0)
     p1 = sl.calibrate(sl.preset("config"), g1.R, g1.m, velocity=V_CUSTOM, cor=0.64)
     _check(g1, p1, 60, 20, hip)
     # a smaller system next: the buffer still holds the first call's partials
