@@ -134,3 +134,12 @@ def test_static_entry_rejects_oversize_and_bad_arguments():
     assert L.sl_step_ordered_static(C.byref(p.replace(precision=sl.PREC_MIXED)), 10, 64, fake, 0, 1, 0, None,
                                     None) == -1
     assert L.sl_step_ordered_static(None, 10, 64, fake, 0, 1, 0, None, None) == -1
+
+
+def test_static_entry_at_capacity():
+    """The largest system the entry point takes, SL_STATIC_MAX_BALLS = 65,536 spheres (its
+    scratch buffer full): two steps, the second a write step, bitwise sl_step's."""
+    hip = _Hip()
+    g = sl.generate_bpca(65536, seed=0x5EED0A03, v_sigma=2.0)
+    p = sl.calibrate(sl.preset("config"), g.R, g.m, velocity=V_CUSTOM, cor=0.64)
+    _check(g, p, 2, 2, hip)
