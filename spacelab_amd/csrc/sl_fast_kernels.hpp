@@ -115,7 +115,8 @@ struct TileArgs {
     const int4* tiles;      // {I, J, system, first block of the system}
     float L0inv, L0inv2;    // mixed: 1 / L0 and 1 / L0^2 (scaled units)
     double Kf;              // mixed: force = Kf * (scaled binary32 factor) * (r / L0)
-    double* part;           // [slot][xyz][n_pad]
+    double* part;           // [slot][xyz][n_pad]; the mixed mode stores binary32 sums, Kf left out,
+                            // in the first half of the same buffer ([slot][xyz][n_pad] floats)
     unsigned long long* nbm;   // [slot][n_pad]: near-partner masks
     double* pe_tile;        // per tile: far-field pair energy (write steps)
     int* queue;             // k_tiles_q: next group per segment, workgroups done (all 0 between launches)
@@ -383,9 +384,9 @@ __device__ __forceinline__ void tile_pass(const TileArgs& a, int tbase)
                 fby = rotk(fby, back);
                 fbz = rotk(fbz, back);
             }
-            const double K = a.Kf;
-            ax = K * (double)fax; ay = K * (double)fay; az = K * (double)faz;
-            bx = K * (double)fbx; by = K * (double)fby; bz = K * (double)fbz;
+            // binary32 sums, exact in binary64 (Kf is applied in k_ball)
+            ax = fax; ay = fay; az = faz;
+            bx = fbx; by = fby; bz = fbz;
         } else if constexpr (PREC == SL_PREC_MIXED) {
             const float qif = (float)qi, gif = (float)gi, L0inv2 = a.L0inv2, L0inv = a.L0inv;
             const float r4i = 4.0f * (float)Ri * L0inv;
@@ -456,9 +457,9 @@ __device__ __forceinline__ void tile_pass(const TileArgs& a, int tbase)
                 fby = rotk(fby, back);
                 fbz = rotk(fbz, back);
             }
-            const double K = a.Kf;
-            ax = K * (double)fax; ay = K * (double)fay; az = K * (double)faz;
-            bx = K * (double)fbx; by = K * (double)fby; bz = K * (double)fbz;
+            // binary32 sums, exact in binary64 (Kf is applied in k_ball)
+            ax = fax; ay = fay; az = faz;
+            bx = fbx; by = fby; bz = fbz;
         } else {
             auto pair = [&](int s, bool act, auto ntest) {
                 const JRec64& r = rec[s];
@@ -555,28 +556,32 @@ __device__ __forceinline__ void tile_pass(const TileArgs& a, int tbase)
     __syncthreads();
     if (!(live && h == 0)) return;
     const int w0 = tl * SPLIT;
-    double v[6];
-    for (int k = 0; k < 6; k++) {
-        double acc = comb[w0][k][l];
-        for (int q = 1; q < SPLIT; q++) acc = acc + comb[w0 + q][k][l];
-        v[k] = acc;
-    }
     const size_t np = (size_t)a.n_pad;
     const size_t si = (size_t)(J - blk0), sj_ = (size_t)(I - blk0);
     const unsigned long long mi = im[tl][l], mj = jm[tl][l];
+    // The mixed mode's partials are binary32: the wavefronts' sums (binary32 values, held
+    // exactly in comb) are added in binary32 and stored in the first half of the buffer.
+    using PT = std::conditional_t<PREC == SL_PREC_MIXED, float, double>;
+    PT v[6];
+    for (int k = 0; k < 6; k++) {
+        PT acc = (PT)comb[w0][k][l];
+        for (int q = 1; q < SPLIT; q++) acc = acc + (PT)comb[w0 + q][k][l];
+        v[k] = acc;
+    }
+    PT* part = reinterpret_cast<PT*>(a.part);
     if (diag) {
         // the i side and the j side of the diagonal tile land on the same balls
-        a.part[(si * 3 + 0) * np + i] = v[0] + v[3];
-        a.part[(si * 3 + 1) * np + i] = v[1] + v[4];
-        a.part[(si * 3 + 2) * np + i] = v[2] + v[5];
+        part[(si * 3 + 0) * np + i] = v[0] + v[3];
+        part[(si * 3 + 1) * np + i] = v[1] + v[4];
+        part[(si * 3 + 2) * np + i] = v[2] + v[5];
         a.nbm[si * np + i] = mi | mj;
     } else {
-        a.part[(si * 3 + 0) * np + i] = v[0];
-        a.part[(si * 3 + 1) * np + i] = v[1];
-        a.part[(si * 3 + 2) * np + i] = v[2];
-        a.part[(sj_ * 3 + 0) * np + j0] = v[3];
-        a.part[(sj_ * 3 + 1) * np + j0] = v[4];
-        a.part[(sj_ * 3 + 2) * np + j0] = v[5];
+        part[(si * 3 + 0) * np + i] = v[0];
+        part[(si * 3 + 1) * np + i] = v[1];
+        part[(si * 3 + 2) * np + i] = v[2];
+        part[(sj_ * 3 + 0) * np + j0] = v[3];
+        part[(sj_ * 3 + 1) * np + j0] = v[4];
+        part[(sj_ * 3 + 2) * np + j0] = v[5];
         a.nbm[si * np + i] = mi;
         a.nbm[sj_ * np + j0] = mj;
     }
@@ -694,7 +699,8 @@ struct BallArgs {
     const int* sys_n;
     const int* sys_nb;
     int n_used, n_pad;
-    const double* part;
+    const double* part;     // [slot][xyz][n_pad]; mixed: binary32, to be scaled by Kf
+    double Kf;
     const unsigned long long* nbm;
     const int* perm;    // engine slot -> the caller's ball index (fixes near-pair orientation)
     int kick, drift_next;
@@ -731,7 +737,8 @@ constexpr int kb_threads(int nwv) { return nwv * TB; }
 // classes c = w, w + NWV, ...
 //
 // 1. For each of its classes a wavefront sums the far-field partials in slot order
-//    (reads of 64 consecutive doubles; Kahan-compensated in the mixed mode), counts
+//    (reads of 64 consecutive doubles; in the mixed mode binary32 sums scaled by Kf
+//    and added with Kahan compensation), counts
 //    the near-partner bits and keeps the first MC nonzero partner masks of each ball
 //    in LDS (more than that: they are read again).
 // 2. The block's near entries are numbered ball by ball (then by class, slot and bit),
@@ -767,6 +774,9 @@ __global__ __launch_bounds__(kb_threads(NWV)) void k_ball(BallArgs a)
     const Phys& P = a.sysP[sys];
     const State& c = a.cur;
     const size_t np = (size_t)a.n_pad;
+    // far-field partials: binary32 in the mixed mode (the force is Kf times the stored sum)
+    using PT = std::conditional_t<COMP, float, double>;
+    const double Kf = a.Kf;
 
     // wavefront 0 finishes the step: its own-state loads (and the block's running pair
     // counters) go out first
@@ -792,13 +802,13 @@ __global__ __launch_bounds__(kb_threads(NWV)) void k_ball(BallArgs a)
         int cn = 0;                    // near entries of this (class, ball)
         int nm = 0;                    // slots of this (class, ball) with near partners
         for (int K0 = cl; K0 < nblk; K0 += SLOTS_IN_FLIGHT * KB_C) {
-            double q[SLOTS_IN_FLIGHT][3];
+            PT q[SLOTS_IN_FLIGHT][3];
             unsigned long long mk[SLOTS_IN_FLIGHT];
 #pragma unroll
             for (int u = 0; u < SLOTS_IN_FLIGHT; u++) {
                 const int K = K0 + u * KB_C;
                 const int Kc = K < nblk ? K : 0;
-                const double* pp = a.part + (size_t)Kc * 3 * np + ic;
+                const PT* pp = reinterpret_cast<const PT*>(a.part) + (size_t)Kc * 3 * np + ic;
                 q[u][0] = pp[0];
                 q[u][1] = pp[np];
                 q[u][2] = pp[2 * np];
@@ -817,9 +827,9 @@ __global__ __launch_bounds__(kb_threads(NWV)) void k_ball(BallArgs a)
                 }
                 if (COMP) {
                     double y, t;
-                    y = q[u][0] - cx; t = fx + y; cx = (t - fx) - y; fx = t;
-                    y = q[u][1] - cy; t = fy + y; cy = (t - fy) - y; fy = t;
-                    y = q[u][2] - cz; t = fz + y; cz = (t - fz) - y; fz = t;
+                    y = Kf * (double)q[u][0] - cx; t = fx + y; cx = (t - fx) - y; fx = t;
+                    y = Kf * (double)q[u][1] - cy; t = fy + y; cy = (t - fy) - y; fy = t;
+                    y = Kf * (double)q[u][2] - cz; t = fz + y; cz = (t - fz) - y; fz = t;
                 } else {
                     fx = fx + q[u][0];
                     fy = fy + q[u][1];

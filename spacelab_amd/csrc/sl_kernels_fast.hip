@@ -303,6 +303,7 @@ int launch_ball_step(FastBuffers& fb, bool write, bool kick, bool drift_next, hi
     a.n_used = fb.lay.n_used;
     a.n_pad = fb.n_pad;
     a.part = fb.part;
+    a.Kf = fb.Kf;
     a.nbm = fb.nbm;
     a.perm = fb.perm;
     a.kick = kick;
