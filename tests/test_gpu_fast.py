@@ -365,11 +365,11 @@ def test_benchmark_aggregate_5000_fast_and_mixed(oracle_mod):
 def test_mixed_against_oracle_5000(oracle_mod):
     """The mixed path against the oracle (the reference's single-core arithmetic) on the
     benchmark's own workload, the 5000-sphere aggregate: after one step the
-    accelerations to 1e-6 (binary32 far field, binary64 contacts; measured 2.0e-7, spins
-    3.2e-8), after 100 steps the positions to 1e-6 (measured 1.8e-7) and velocities to
-    2e-3 (4.4e-4: the one-step difference grows with the aggregate's e-folding time of
+    accelerations to 1e-6 (binary32 far field, binary64 contacts; measured 1.6e-7, spins
+    3.6e-8), after 100 steps the positions to 1e-6 (measured 1.1e-7) and velocities to
+    2e-3 (7.5e-4: the one-step difference grows with the aggregate's e-folding time of
     about five steps) of their largest values, and the energies at the write steps to
-    1e-6 of the energy scale (1.2e-10 and 1.4e-8)."""
+    1e-6 of the energy scale (1.5e-10 and 4.9e-8)."""
     n = 5000
     g = sl.generate_bpca(n, seed=0x5EED0001, v_sigma=1e-3)
     p = sl.calibrate(sl.preset("config"), g.R, g.m, velocity=V_CUSTOM, cor=0.64, dt_rule="tc50").replace(
