@@ -764,7 +764,9 @@ __global__ __launch_bounds__(kb_threads(NWV)) void k_ball(BallArgs a)
     __shared__ NearRes res[KB_T];
     __shared__ unsigned long long s_mk[KB_C][KB_MC][TB];   // nonzero partner masks ...
     __shared__ int s_mks[KB_C][KB_MC][TB];                  // ... and their slots
-    const int w = threadIdx.x / TB, l = threadIdx.x % TB;
+    // the wavefront index is uniform: in a scalar register, the slot indices derived from
+    // it are too
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / TB), l = threadIdx.x % TB;
     const int i = blockIdx.x * TB + l;
     const int sys = a.blk_sys[blockIdx.x];
     const int off = a.sys_off[sys];
