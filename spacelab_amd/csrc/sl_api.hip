@@ -313,6 +313,13 @@ int tune_ordered(sl_ctx* c)
     const Phys P = phys_of(c->p);
     const int ns = ordered_shapes();
     std::vector<float> t(ns, 1e30f);
+    // two events of the pool, handed back on every way out
+    const size_t ev_mark = c->ev_used;
+    struct EvRelease {
+        sl_ctx* c;
+        size_t mark;
+        ~EvRelease() { c->ev_used = mark; }
+    } release{c, ev_mark};
     hipEvent_t e0 = next_event(c), e1 = next_event(c);
     if (!e0 || !e1) return set_err(c, SL_E_HIP, "tuner events");
     for (int s = 0; s < ns; s++) {
@@ -323,7 +330,6 @@ int tune_ordered(sl_ctx* c)
         HIPCHK(c, hipEventSynchronize(e1));
         HIPCHK(c, hipEventElapsedTime(&t[s], e0, e1));
     }
-    c->ev_used -= 2;
     int best = 0;
     for (int s = 1; s < ns; s++)
         if (t[s] < t[best]) best = s;
