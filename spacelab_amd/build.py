@@ -73,7 +73,7 @@ def build(verbose: bool = False) -> str:
     for src, exe in PROGRAMS:
         out = os.path.join(LIBDIR, exe)
         cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
-               "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + HOSTDIR, os.path.join(HOSTDIR, src),
+               "-Wall", "-pthread", "-I" + os.path.join(ROOT, "include"), "-I" + HOSTDIR, os.path.join(HOSTDIR, src),
                "-L" + LIBDIR, "-lspacelab", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

@@ -19,12 +19,17 @@
 #include "spacelab/sl_csv.h"
 
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <ctime>
+#include <deque>
+#include <exception>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace spacelab {
@@ -198,6 +203,101 @@ struct RunControl {
     }
 };
 
+// simData rows formatted on a host thread of their own, in the order they were queued,
+// while the GPU runs the next steps: a write step hands over a copy of the state and
+// returns.  take() waits for the queue to drain and returns the text.
+class RowFormatter {
+public:
+    RowFormatter() : th_([this] { run(); }) {}
+    ~RowFormatter()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        th_.join();
+    }
+    RowFormatter(const RowFormatter&) = delete;
+    RowFormatter& operator=(const RowFormatter&) = delete;
+
+    void push(int n, const double* pos3, const double* w3, const double* vel3)
+    {
+        Job j{n, std::vector<double>(pos3, pos3 + 3 * (size_t)n), std::vector<double>(w3, w3 + 3 * (size_t)n),
+              std::vector<double>(vel3, vel3 + 3 * (size_t)n)};
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            q_.push_back(std::move(j));
+        }
+        cv_.notify_all();
+    }
+
+    // the rows of every job queued so far, each preceded by '\n'
+    std::string take()
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [this] { return q_.empty() && !busy_; });
+        if (err_) {
+            std::exception_ptr e = err_;
+            err_ = nullptr;
+            std::rethrow_exception(e);
+        }
+        std::string out;
+        out.swap(out_);
+        return out;
+    }
+
+private:
+    struct Job {
+        int n;
+        std::vector<double> pos, w, vel;
+    };
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<Job> q_;
+    std::string out_;
+    bool busy_ = false, stop_ = false;
+    std::exception_ptr err_;
+    std::thread th_;
+
+    void run()
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
+            if (q_.empty()) return;   // stop_ with nothing left
+            Job j = std::move(q_.front());
+            q_.pop_front();
+            busy_ = true;
+            lk.unlock();
+            std::string row;
+            std::exception_ptr e;
+            try {
+                // a number takes at most 13 characters at 6 digits ("-1.23457e-308")
+                row.assign((size_t)j.n * (11 * 14) + 16, '\0');
+                long long len = sl_csv_sim_row(j.n, j.pos.data(), j.w.data(), j.vel.data(), &row[0], (long long)row.size());
+                if (len >= (long long)row.size()) {
+                    row.assign((size_t)len + 1, '\0');
+                    len = sl_csv_sim_row(j.n, j.pos.data(), j.w.data(), j.vel.data(), &row[0], (long long)row.size());
+                }
+                if (len < 0) throw Error((int)len, "CSV formatting");
+                row.resize((size_t)len);
+            } catch (...) {
+                e = std::current_exception();
+            }
+            lk.lock();
+            if (e) {
+                if (!err_) err_ = e;
+            } else {
+                out_ += '\n';
+                out_ += row;
+            }
+            busy_ = false;
+            cv_.notify_all();
+        }
+    }
+};
+
 class Collider {
 public:
     std::string output_prefix;
@@ -312,6 +412,7 @@ public:
     // loop, so rows buffered after its last flush are not written by it.
     void flush()
     {
+        ballBuffer += rows.take();
         append(output_prefix + "simData.csv", ballBuffer);
         append(output_prefix + "energy.csv", energyBuffer);
         ballBuffer.clear();
@@ -332,6 +433,7 @@ private:
     sl_ctx* ctx = nullptr;
     double dt = 0;
     std::string ballBuffer, energyBuffer;
+    RowFormatter rows;
     time_t lastWrite = 0;
 
     void check(int rc, const char* what)
@@ -348,8 +450,8 @@ private:
         check(sl_step(ctx, 1, 1, &e, 1), "sl_step");
         add_energy(e);
         download();
-        ballBuffer += '\n';
-        ballBuffer += sim_row();
+        // the row is formatted on the formatter's thread while the next steps run
+        rows.push(O.num_particles, &O.pos[0].x, &O.w[0].x, &O.vel[0].x);
     }
 
     void add_energy(const sl_energy& e)
