@@ -184,11 +184,13 @@ void k_pairs_ordered(State s, Phys P, int n, int write, double* __restrict__ pe_
     __shared__ unsigned long long cmask[2][BI][MW];
     __shared__ PartnerRec sp[2][CJ];                 // the partners of chunks k and k + 1
     __shared__ unsigned ccount;
+    __shared__ double sgm[BI];                       // G m_i of the workgroup's balls
 
     const int t = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(t / 64);
     if (t < 2 * BI * MW) (&cmask[0][0][0])[t] = 0ull;
     if (t == 0) ccount = 0;
+    if (t < BI) sgm[t] = blockIdx.x * BI + t < n ? P.G * s.f[MASS][blockIdx.x * BI + t] : 0.0;
 
     // producer role
     const int il = t % BI;
@@ -309,7 +311,16 @@ void k_pairs_ordered(State s, Phys P, int n, int write, double* __restrict__ pe_
                         const double yd = 1.0 / dist;
                         const D3 rh = d3(div_rcp(r.x, dist, yd), div_rcp(r.y, dist, yd), div_rcp(r.z, dist, yd));
                         D3 F = rh * fv;
-                        if (P.far_gravity) F = F + rh * ((IA ? Gmi * mj : P.G * mj * mi) / (dist * dist));
+                        if (P.far_gravity) {
+                            // Under a register cap G m_i is read from LDS at each use (an atomic
+                            // load, which the compiler keeps in the loop): held in a register it
+                            // is spilled, and the spill's reload waits for every load in flight,
+                            // the next chunk's partners included.
+                            double gmi = Gmi;
+                            if constexpr (WPE > 0)
+                                gmi = __hip_atomic_load(&sgm[il], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            F = F + rh * ((IA ? gmi * mj : P.G * mj * mi) / (dist * dist));
+                        }
                         // acc[A] += F/m[A]  or  acc[B] -= F/m[B]   (:187-188, 249-250)
                         c0 = div_rcp(F.x, mi, ymi);
                         c1 = div_rcp(F.y, mi, ymi);
