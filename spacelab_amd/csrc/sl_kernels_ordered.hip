@@ -477,7 +477,7 @@ struct OrdShape {
     int bi, npw, cj, ded;
 };
 constexpr OrdShape ORD_SHAPES[] = {
-    {4, 4, 128, 0}, {4, 3, 48, 1}, {4, 4, 64, 0}, {8, 4, 32, 0}, {2, 2, 128, 0},
+    {2, 3, 192, 1}, {4, 3, 48, 1}, {4, 4, 64, 0}, {8, 4, 32, 0}, {2, 2, 128, 0},
     {4, 3, 96, 1},  {2, 2, 128, 0}, {2, 4, 128, 0}, {2, 4, 128, 0},
 };
 constexpr int N_ORD_SHAPES = (int)(sizeof(ORD_SHAPES) / sizeof(ORD_SHAPES[0]));
@@ -506,7 +506,7 @@ hipError_t launch_pairs_ordered(const State& s, const Phys& P, int n, int write,
     // template arguments: balls, producer wavefronts, partners per chunk, dedicated consumer
     // wavefront, contact arm out of line, wavefronts per SIMD the registers are held to
     switch (shape) {
-    case 0: k_pairs_ordered<4, 4, 128, false, false, 4><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
+    case 0: k_pairs_ordered<2, 3, 192, true, false, 3><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
     case 1: k_pairs_ordered<4, 3, 48, true, false, 3><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
     case 2: k_pairs_ordered<4, 4, 64, false, false, 3><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
     case 3: k_pairs_ordered<8, 4, 32, false, false, 3><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
