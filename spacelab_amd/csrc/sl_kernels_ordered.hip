@@ -486,10 +486,11 @@ constexpr int N_ORD_SHAPES = (int)(sizeof(ORD_SHAPES) / sizeof(ORD_SHAPES[0]));
 int ordered_shapes() { return N_ORD_SHAPES; }
 
 // The shape used when none is chosen (the allocation-free entry point, and until the
-// tuner has run), from timings on MI355X: up to about 1500 balls 2 balls per workgroup of
-// four wavefronts and chunks of 128 partners; above, 4 balls, chunks of 64 and registers
-// held to 3 wavefronts per SIMD.
-int ordered_default_shape(int n) { return n < 1500 ? 8 : 2; }
+// tuner has run), from timings on MI355X, both with a consumer wavefront of their own and
+// registers held to 3 wavefronts per SIMD: up to about 1500 balls 2 balls per workgroup and
+// chunks of 192 partners; above, 4 balls and chunks of 96 (the fastest shape from 2,200 to
+// 65,536 balls).
+int ordered_default_shape(int n) { return n < 1500 ? 0 : 5; }
 
 int ordered_blocks(int n, int shape)
 {
