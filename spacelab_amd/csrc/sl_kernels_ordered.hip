@@ -165,6 +165,8 @@ __device__ __attribute__((noinline)) void contact_pair(const double* base, long 
 
 // CALL: the contact arm out of line (its registers weigh only on the calls) or inline.
 // WPE: the wavefronts per SIMD the register budget is held to (0: the compiler's choice).
+// UNR: how many of a thread's far-field pairs per chunk the sweep unrolls (the sums' order
+// is fixed by the contribution buffer, so it changes no bits).
 template <int BI, int NPW, int CJ, bool DED, bool CALL, int WPE, int UNR = 1>
 __global__ __launch_bounds__((NPW + (DED ? 1 : 0)) * 64) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_pairs_ordered(State s, Phys P, int n, int write, double* __restrict__ pe_ball,
@@ -505,14 +507,16 @@ hipError_t launch_pairs_ordered(const State& s, const Phys& P, int n, int write,
     if (shape < 0 || shape >= N_ORD_SHAPES) shape = 0;
     const int nb = ordered_blocks(n, shape);
     // template arguments: balls, producer wavefronts, partners per chunk, dedicated consumer
-    // wavefront, contact arm out of line, wavefronts per SIMD the registers are held to
+    // wavefront, contact arm out of line, wavefronts per SIMD the registers are held to,
+    // unroll of the far-field loop (case 5: both of a thread's pairs per chunk, 1% faster at
+    // 5,000 balls than one at a time)
     switch (shape) {
     case 0: k_pairs_ordered<2, 3, 192, true, false, 3><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
     case 1: k_pairs_ordered<4, 3, 48, true, false, 3><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
     case 2: k_pairs_ordered<4, 4, 64, false, false, 3><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
     case 3: k_pairs_ordered<8, 4, 32, false, false, 3><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
     case 4: k_pairs_ordered<2, 2, 128, false, false, 0><<<nb, 128, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
-    case 5: k_pairs_ordered<4, 3, 96, true, false, 3><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
+    case 5: k_pairs_ordered<4, 3, 96, true, false, 3, 2><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
     case 6: k_pairs_ordered<2, 2, 128, false, true, 0><<<nb, 128, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
     case 7: k_pairs_ordered<2, 4, 128, false, false, 3><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
     default: k_pairs_ordered<2, 4, 128, false, false, 0><<<nb, 256, 0, st>>>(s, P, n, write, pe_ball, contacts); break;
