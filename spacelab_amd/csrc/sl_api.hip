@@ -242,9 +242,10 @@ void clear_ord_graphs(sl_ctx* c)
 
 constexpr int ORD_GRAPH_MAX_STEPS = 128;
 
-// len ordered steps without energies (drift, pair pass, closing kick each), as one
-// replay of a graph captured on first use: the same kernels with the same arguments as
-// the direct launches, so the same bits.
+// len ordered steps without energies, as one replay of a graph captured on first use:
+// drift, then per step the pair pass and the closing kick, each kick but the last fused
+// with the next step's drift (the same operations as the direct launches, so the same
+// bits, with one launch fewer per step).
 int ordered_run(sl_ctx* c, int len)
 {
     hipGraphExec_t ex = nullptr;
@@ -255,11 +256,11 @@ int ordered_run(sl_ctx* c, int len)
         const Phys P = phys_of(c->p);
         if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
             return set_err(c, SL_E_HIP, "ordered step capture");
-        bool ok = true;
+        bool ok = launch_drift(c->st, c->n, c->p.dt, c->stream) == hipSuccess;
         for (int k = 0; k < len && ok; k++)
-            ok = launch_drift(c->st, c->n, c->p.dt, c->stream) == hipSuccess &&
-                 launch_pairs_ordered(c->st, P, c->n, 0, c->pe_part, c->cnt, c->stream, c->ord_shape) == hipSuccess &&
-                 launch_kick(c->st, c->n, c->p.dt, 0, 1, c->e_part, c->nonfinite, c->stream) == hipSuccess;
+            ok = launch_pairs_ordered(c->st, P, c->n, 0, c->pe_part, c->cnt, c->stream, c->ord_shape) == hipSuccess &&
+                 (k + 1 < len ? launch_kick_drift(c->st, c->n, c->p.dt, c->nonfinite, c->stream)
+                              : launch_kick(c->st, c->n, c->p.dt, 0, 1, c->e_part, c->nonfinite, c->stream)) == hipSuccess;
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (!ok || e != hipSuccess || !g) {

@@ -23,6 +23,9 @@ hipError_t launch_pairs_ordered(const State& s, const Phys& P, int n, int write,
 int kick_blocks(int n);
 hipError_t launch_kick(const State& s, int n, double dt, int write, int kick, double* e_part,
                        int* nonfinite, hipStream_t st);
+// A step's closing kick (no energies) and the next step's drift in one launch: the same
+// operations as launch_kick then launch_drift, so the same bits.
+hipError_t launch_kick_drift(const State& s, int n, double dt, int* nonfinite, hipStream_t st);
 
 // ---- small utilities: sl_util.hip
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t st);

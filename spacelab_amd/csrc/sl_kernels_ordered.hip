@@ -456,6 +456,51 @@ __global__ __launch_bounds__(256) void k_kick(State s, int n, double dt, int wri
     }
 }
 
+// The closing kick of a step without energies (k_kick with kick = 1, write = 0) and the
+// next step's drift (k_drift) in one pass over the balls: the velocities the kick writes
+// are the ones the drift reads, and the accelerations are the same pair pass's, so every
+// value is computed by the same operations as in the two launches.
+__global__ void k_kick_drift(State s, int n, double dt, int* __restrict__ nonfinite)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const double ax = s.f[AX][b], ay = s.f[AY][b], az = s.f[AZ][b];
+    const double bx = s.f[BX][b], by = s.f[BY][b], bz = s.f[BZ][b];
+    // kick (ColliderSingleCore.cpp:263-287)
+    const double vx = s.f[HX][b] + .5 * ax * dt;
+    const double vy = s.f[HY][b] + .5 * ay * dt;
+    const double vz = s.f[HZ][b] + .5 * az * dt;
+    const double wx = s.f[GX][b] + .5 * bx * dt;
+    const double wy = s.f[GY][b] + .5 * by * dt;
+    const double wz = s.f[GZ][b] + .5 * bz * dt;
+    s.f[VX][b] = vx;
+    s.f[VY][b] = vy;
+    s.f[VZ][b] = vz;
+    s.f[WX][b] = wx;
+    s.f[WY][b] = wy;
+    s.f[WZ][b] = wz;
+    const double px = s.f[PX][b], py = s.f[PY][b], pz = s.f[PZ][b];
+    if (!(isfinite(vx) && isfinite(vy) && isfinite(vz) && isfinite(wx) && isfinite(wy) &&
+          isfinite(wz) && isfinite(px) && isfinite(py) && isfinite(pz)))
+        atomicOr(nonfinite, 1);
+    // drift of the next step (ColliderSingleCore.cpp:82-97)
+    s.f[QX][b] = px;
+    s.f[QY][b] = py;
+    s.f[QZ][b] = pz;
+    const double hx = vx + .5 * ax * dt;
+    const double hy = vy + .5 * ay * dt;
+    const double hz = vz + .5 * az * dt;
+    s.f[HX][b] = hx;
+    s.f[HY][b] = hy;
+    s.f[HZ][b] = hz;
+    s.f[GX][b] = wx + .5 * bx * dt;
+    s.f[GY][b] = wy + .5 * by * dt;
+    s.f[GZ][b] = wz + .5 * bz * dt;
+    s.f[PX][b] = px + hx * dt;
+    s.f[PY][b] = py + hy * dt;
+    s.f[PZ][b] = pz + hz * dt;
+}
+
 // ---- launchers --------------------------------------------------------------
 
 hipError_t launch_drift(const State& s, int n, double dt, hipStream_t st)
@@ -525,6 +570,13 @@ hipError_t launch_pairs_ordered(const State& s, const Phys& P, int n, int write,
 }
 
 int kick_blocks(int n) { return (n + 255) / 256; }
+
+hipError_t launch_kick_drift(const State& s, int n, double dt, int* nonfinite, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    k_kick_drift<<<kick_blocks(n), 256, 0, st>>>(s, n, dt, nonfinite);
+    return hipGetLastError();
+}
 
 hipError_t launch_kick(const State& s, int n, double dt, int write, int kick, double* e_part,
                        int* nonfinite, hipStream_t st)

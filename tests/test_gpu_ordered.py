@@ -299,7 +299,8 @@ def test_ordered_shapes_bitwise(oracle_mod):
 
 
 def test_ordered_graphs_same_bits():
-    """Runs of ordered steps between write steps replay captured graphs; with graphs off
+    """Runs of ordered steps between write steps replay captured graphs, in which each
+    closing kick but the run's last is fused with the next step's drift; with graphs off
     every kernel is launched directly.  Both give the same bits in every state array and
     the same energies, over runs that end on and between write steps and across calls of
     different lengths (so several captured lengths are replayed)."""
