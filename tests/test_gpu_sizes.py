@@ -4,8 +4,11 @@
   block, one over, and two blocks plus one: the fast path pads every system to whole
   64-sphere blocks, so these put padded lanes in the diagonal tile, in a full tile and
   in a tile of one real sphere.
-- 7,519, 7,520 and 15,040 spheres: the ordered pair pass takes 4 spheres per workgroup
-  below 7,520, 8 from there on and 16 from 15,040.
+- 7,519, 7,520 and 15,040 spheres: for the ordered pair pass's launch shapes (2, 4 or 8
+  spheres per workgroup) a ragged last workgroup (7,519) and whole ones, with the last
+  partner chunk cut short; the first step times every shape on them.
+- The 20-step cases replay captured graphs of 18 ordered steps, in which each closing kick
+  but the last is fused with the next step's drift.
 
 The ordered path is bitwise the oracle at every size.  The fast path meets its one-step
 bars (forces and torques to 1e-13 relative, velocities to 1e-14, positions to 1e-15) and,
