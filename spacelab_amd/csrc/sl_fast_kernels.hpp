@@ -215,14 +215,18 @@ struct JRecMix {
     float Rs, q, g, pad;
 };
 static_assert(sizeof(JRecMix) == sizeof(JRec64), "both records share the LDS staging area");
-// The mixed path's record for a far tile (every pair farther apart than S + h_min): all
+// The mixed path's staging for a far tile (every pair farther apart than S + h_min): all
 // binary32, ball j's block-local position shifted by the offset between the two blocks'
-// reference points, so a pair's displacement is one binary32 subtraction per axis.
-struct JRec32 {
-    float4 p;   // x, y, z in units of L0 relative to block I's reference point; R / L0
-    float4 w;   // cohesion and gravity weights (q, g), 0, 0
+// reference points, so a pair's displacement is one binary32 subtraction per axis.  Three
+// arrays rather than one record, so that lane l's reads at step s (slot l + s) are 16 or 4
+// bytes apart from its neighbours': every LDS bank once per pass (a 32-byte record puts
+// two lanes on each bank of a 16-byte read and eight on each of a 4-byte one).
+struct JSoA32 {
+    float4 p[2 * TB];   // x, y, z in units of L0 relative to block I's reference point; R / L0
+    float q[2 * TB];    // cohesion weight
+    float g[2 * TB];    // gravity weight
 };
-static_assert(sizeof(JRec32) <= sizeof(JRec64), "all records share the LDS staging area");
+static_assert(sizeof(JSoA32) <= sizeof(JRec64) * 2 * TB, "all records share the LDS staging area");
 
 // Reference point of a 64-ball block: the centre of its bounding box (lo, hi).
 __device__ __forceinline__ double3 block_ref(double4 lo, double4 hi)
@@ -278,13 +282,17 @@ __device__ __forceinline__ void tile_pass(const TileArgs& a, int tbase)
             const double3 cj = block_ref(a.bnd[2 * J], a.bnd[2 * J + 1]);
             const double L0inv = (double)a.L0inv;
             const float4 xj = a.xl[j0];
-            JRec32 r;
-            r.p = make_float4(xj.x + (float)((cj.x - ci.x) * L0inv), xj.y + (float)((cj.y - ci.y) * L0inv),
-                              xj.z + (float)((cj.z - ci.z) * L0inv), xj.w);
-            r.w = make_float4(a.aux.q32[j0], a.aux.g32[j0], 0.f, 0.f);
-            JRec32* sm = reinterpret_cast<JRec32*>(sj[tl]);
-            sm[l] = r;
-            sm[l + TB] = r;
+            const float4 pj = make_float4(xj.x + (float)((cj.x - ci.x) * L0inv),
+                                          xj.y + (float)((cj.y - ci.y) * L0inv),
+                                          xj.z + (float)((cj.z - ci.z) * L0inv), xj.w);
+            const float qj = a.aux.q32[j0], gj = a.aux.g32[j0];
+            JSoA32* sm = reinterpret_cast<JSoA32*>(sj[tl]);
+            sm->p[l] = pj;
+            sm->p[l + TB] = pj;
+            sm->q[l] = qj;
+            sm->q[l + TB] = qj;
+            sm->g[l] = gj;
+            sm->g[l + TB] = gj;
         } else if constexpr (PREC == SL_PREC_MIXED) {
             JRecMix r;
             r.x = a.px[j0];
@@ -331,7 +339,10 @@ __device__ __forceinline__ void tile_pass(const TileArgs& a, int tbase)
             const float qif = a.aux.q32[i], gif = a.aux.g32[i];
             const float r4i = 4.0f * xi4.w;
             float fax = 0.f, fay = 0.f, faz = 0.f, fbx = 0.f, fby = 0.f, fbz = 0.f;
-            const JRec32* rec32 = reinterpret_cast<const JRec32*>(sj[tl]) + l;
+            const JSoA32* soa = reinterpret_cast<const JSoA32*>(sj[tl]);
+            const float4* rp32 = soa->p + l;
+            const float* rq32 = soa->q + l;
+            const float* rg32 = soa->g + l;
             // Gravity is left out of a tile whose every pair has it below 2^-27 of the vdW
             // term: g_i g_j / d^3 over q_i q_j / (s t)^2 is at most (g/q)_I (g/q)_J dmax^5,
             // because s, t <= d^2 <= dmax^2 (dmax: the largest distance between the blocks'
@@ -348,18 +359,17 @@ __device__ __forceinline__ void tile_pass(const TileArgs& a, int tbase)
                 grav = !(bound < 0x1p-27);
             }
             auto pair32 = [&](int s, auto gv) {
-                const float4 rp = rec32[s].p;
-                const float4 rw = rec32[s].w;
+                const float4 rp = rp32[s];
                 const float dx = rp.x - xi4.x, dy = rp.y - xi4.y, dz = rp.z - xi4.z;
                 const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                 const float S = xi4.w + rp.w;
                 const float sv = __builtin_fmaf(-S, S, d2);
                 const float tv = __builtin_fmaf(r4i, rp.w, sv);
                 const float st = sv * tv;
-                float f = (qif * rw.x) * __builtin_amdgcn_rcpf(st * st);
+                float f = (qif * rq32[s]) * __builtin_amdgcn_rcpf(st * st);
                 if constexpr (decltype(gv)::value) {
                     const float rq = __builtin_amdgcn_rsqf(d2);
-                    f = __builtin_fmaf(gif * rw.y, rq * rq * rq, f);
+                    f = __builtin_fmaf(gif * rg32[s], rq * rq * rq, f);
                 }
                 fax = __builtin_fmaf(f, dx, fax);
                 fay = __builtin_fmaf(f, dy, fay);
